@@ -1,0 +1,188 @@
+/*
+ * mph.h -- C ABI of the MI355X-native MPH-WC explicit solver hot path.
+ *
+ * The reference (source/main.cpp of MphExplicit) has no library API: its "boundary" is the
+ * sequence of static calls inside main() (source/main.cpp:357-452) over file-scope globals
+ * (source/main.cpp:56-156), fed by a .data keyword file and a .grid particle file.  This
+ * header is the drop-in for that loop: the host keeps the reference's formats and the
+ * per-step call order, and every call below replaces a named piece of main.cpp.
+ *
+ * Conventions
+ *   - plain C types only; vectors are [n][3] double (the reference's storage, main.cpp:78-80)
+ *   - every function returns an int status (MPH_OK == 0, negative on error) unless noted;
+ *     mph_last_error() gives a message for the calling thread.  The library never exits
+ *     the process (the reference exits on malloc/fopen failure, source/errorfunc.cpp:8-31).
+ *   - particle types 0,1 are fluid and 2,3 wall (main.cpp:43-47).
+ *   - one context = one device stream; calls on one context are not thread safe.
+ */
+#ifndef MPH_H_INCLUDED
+#define MPH_H_INCLUDED
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPH_ABI_VERSION 1
+
+enum mph_status {
+    MPH_OK = 0,
+    MPH_ERR_ARG = -1,       /* invalid argument / unsupported configuration */
+    MPH_ERR_IO = -2,        /* file could not be opened */
+    MPH_ERR_FORMAT = -3,    /* malformed .grid file */
+    MPH_ERR_DEVICE = -4,    /* HIP runtime error or no usable GPU */
+    MPH_ERR_CAPACITY = -5,  /* a neighbour list outgrew its reserved width */
+    MPH_ERR_STATE = -6      /* call not valid in the context's current state */
+};
+
+enum mph_precision {
+    MPH_FP64 = 0,           /* all pair math and sums in FP64 */
+    MPH_MIXED = 1           /* FP32 pair geometry/force sums; FP64 positions, integration, density/pressure */
+};
+
+enum mph_order {
+    MPH_ORDER_CURRENT = 0,  /* the solver's current (cell-sorted) particle order, as the reference stores it */
+    MPH_ORDER_ORIGINAL = 1  /* by ParticleIndex = line order of the .grid file (main.cpp:645-647) */
+};
+
+/* Everything the reference reads from .data (main.cpp:502-521) and the .grid header
+ * (main.cpp:549-557), plus the simulated dimension (compile-time TWO_DIMENSIONAL in the
+ * reference, main.cpp:38) and the arithmetic precision. */
+typedef struct mph_params {
+    int dim;                          /* 3, or 2 (storage stays 3-component, main.cpp:40) */
+    int precision;                    /* enum mph_precision */
+    double particle_spacing;          /* l0 */
+    double domain_min[3];
+    double domain_max[3];
+    double time;                      /* start time, .grid line 1 */
+    double dt;
+    double output_interval;
+    double vtk_output_interval;
+    double end_time;
+    double radius_ratio_a;            /* also used for the gravity-centre kernel (main.cpp:838) */
+    double radius_ratio_p;
+    double radius_ratio_v;
+    double density[4];
+    double bulk_modulus[4];
+    double bulk_viscosity[4];
+    double shear_viscosity[4];
+    double surface_tension[4];
+    double interaction_ratio[4][4];   /* [type i][type j] */
+    double gravity[3];
+    double wall_center[4][3];         /* only entries 2,3 are used */
+    double wall_velocity[4][3];
+    double wall_omega[4][3];
+} mph_params;
+
+/* Host output buffers for mph_get_fields; any pointer may be NULL. */
+typedef struct mph_fields {
+    int *particle_index;              /* [n] original id (ParticleIndex, main.cpp:75) */
+    int *type;                        /* [n] Property */
+    double *position;                 /* [n][3] */
+    double *velocity;                 /* [n][3] */
+    double *force;                    /* [n][3] total force of the last step (main.cpp:80) */
+    double *density_a;                /* [n] DensityA (main.cpp:131) */
+    double *gravity_center;           /* [n][3] GravityCenter (main.cpp:132) */
+    double *pressure_a;               /* [n] PressureA */
+    double *vol_strain_p;             /* [n] VolStrainP */
+    double *divergence_p;             /* [n] DivergenceP */
+    double *pressure_p;               /* [n] PressureP */
+    double *virial_pressure;          /* [n] VirialPressureAtParticle (valid after mph_compute_virial) */
+    int *neighbor_count;              /* [n] NeighborCount: list length incl. self (main.cpp:81) */
+} mph_fields;
+
+typedef struct mph_stats {
+    long long steps;                  /* steps taken since create */
+    long long rebuilds;               /* neighbour-list rebuilds (incl. the initial one) */
+    long long list_entries;           /* stored neighbour entries at the last rebuild (no self) */
+    int list_width;                   /* reserved entries per particle */
+    int max_list_len;                 /* longest list at the last rebuild */
+} mph_stats;
+
+typedef struct mph_ctx mph_ctx;
+
+/* ---- host-side formats (no device needed) ------------------------------------------ */
+
+/* Reference defaults of the .data statics (main.cpp:56-211): zeros, Dt = 1e100. */
+void mph_params_default(mph_params *p);
+
+/* readDataFile, main.cpp:489-539.  Unknown lines are ignored (the reference logs them);
+ * CRLF line ends, '#' comments and leading blanks are accepted. */
+int mph_read_data_file(const char *path, mph_params *p);
+
+/* readGridFile header, main.cpp:549-557: Time, count, spacing, domain. */
+int mph_read_grid_header(const char *path, mph_params *p, int *count);
+
+/* readGridFile body, main.cpp:632-639: type, position, velocity of `count` particles. */
+int mph_read_grid_particles(const char *path, int count, int *type, double *x, double *v);
+
+/* writeProfFile, main.cpp:672-696 (the .grid layout with %e). */
+int mph_write_prof_file(const char *path, const mph_params *p, double time, int n,
+                        const int *type, const double *x, const double *v);
+
+/* writeVtkFile, main.cpp:698-834 (legacy ASCII VTK, float casts, same field order). */
+int mph_write_vtk_file(const char *path, int n, const int *type, const double *x,
+                       const double *v, const double *density_a, const double *vol_strain_p,
+                       const double *virial_pressure, const int *neighbor_count,
+                       const double *force);
+
+/* ---- device path -------------------------------------------------------------------- */
+
+/* Number of visible GPUs (0 when none); never fails. */
+int mph_device_count(void);
+
+/* Upload one case and run the reference's set-up: initializeWeight/Fluid/Wall/Domain
+ * (main.cpp:330-333), then the first bucketing, neighbour build and densities
+ * (main.cpp:358-364).  Returns NULL on error (see mph_last_error). */
+mph_ctx *mph_create(const mph_params *p, int n, const int *type, const double *x,
+                    const double *v);
+
+/* Same for `ncases` independent cases run side by side in every launch (no reference
+ * counterpart: the reference runs one case per process).  Particle arrays are the cases'
+ * arrays concatenated; counts[c] particles belong to case c. */
+mph_ctx *mph_create_batch(int ncases, const mph_params *p, const int *counts, const int *type,
+                          const double *x, const double *v);
+
+/* `nsteps` iterations of the reference's loop body (main.cpp:386-431) for every case:
+ * convection, walls, periodic wrap, Verlet check + rebuild, densities, pressures,
+ * surface tension, viscosity, gravity, acceleration; Time += Dt (main.cpp:449).
+ * Asynchronous on the context's stream except for a capacity check at the end. */
+int mph_step(mph_ctx *ctx, int nsteps);
+
+/* Capture one step into a HIP graph and replay it for `nsteps` (same results as mph_step). */
+int mph_step_graph(mph_ctx *ctx, int nsteps);
+
+/* Wait for all queued work of the context. */
+int mph_sync(mph_ctx *ctx);
+
+/* Number of particles of case `case_id` (or of all cases for case_id = -1). */
+int mph_particle_count(const mph_ctx *ctx, int case_id);
+
+/* Simulated time of a case. */
+int mph_get_time(const mph_ctx *ctx, int case_id, double *time);
+
+/* Copy per-particle fields of a case to the host (synchronises). */
+int mph_get_fields(mph_ctx *ctx, int case_id, int order, mph_fields *out);
+
+/* calculateVirialStressAtParticle, main.cpp:2057-2302, for the current state (the
+ * reference calls it before each .vtk write, main.cpp:436-437).  Read the result with
+ * mph_get_fields(...virial_pressure...). */
+int mph_compute_virial(mph_ctx *ctx);
+
+/* Neighbour lists of the last rebuild in CSR form keyed by original particle id:
+ * ptr[n+1], ind[total] with each row ascending and self included, i.e. the reference's
+ * NeighborPtr/NeighborInd (main.cpp:1329-1596) mapped through ParticleIndex.  Call with
+ * ind == NULL to get only ptr (ptr[n] = total). */
+int mph_get_neighbors(mph_ctx *ctx, int case_id, long long *ptr, int *ind);
+
+int mph_get_stats(const mph_ctx *ctx, mph_stats *out);
+
+void mph_destroy(mph_ctx *ctx);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *mph_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPH_H_INCLUDED */
