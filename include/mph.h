@@ -95,8 +95,18 @@ typedef struct mph_stats {
     long long rebuilds;               /* neighbour-list rebuilds (incl. the initial one) */
     long long list_entries;           /* stored neighbour entries at the last rebuild (no self) */
     int list_width;                   /* reserved entries per particle */
-    int max_list_len;                 /* longest list at the last rebuild */
+    int max_list_len;                 /* longest list (without self) since create */
 } mph_stats;
+
+/* Per-phase device time of mph_step_timed, summed over its steps. */
+typedef struct mph_timing {
+    double move_ms;                   /* convection, walls, wrap, Verlet check */
+    double rebuild_ms;                /* bucketing + lists (near zero on steps without rebuild) */
+    double sweep_a_ms;                /* densities, divergence, pressures */
+    double sweep_b_ms;                /* forces and velocity update */
+    long long pairs;                  /* sum over steps of ordered pairs i != j closer than the largest kernel radius */
+    long long rebuilds;               /* rebuilds during these steps (max over cases) */
+} mph_timing;
 
 typedef struct mph_ctx mph_ctx;
 
@@ -150,6 +160,11 @@ int mph_step(mph_ctx *ctx, int nsteps);
 
 /* Capture one step into a HIP graph and replay it for `nsteps` (same results as mph_step). */
 int mph_step_graph(mph_ctx *ctx, int nsteps);
+
+/* mph_step with HIP events around each phase on the context's stream (no reference
+ * counterpart: the reference only keeps process CPU-time buckets, main.cpp:383-451).
+ * Synchronises once at the end. */
+int mph_step_timed(mph_ctx *ctx, int nsteps, mph_timing *out);
 
 /* Wait for all queued work of the context. */
 int mph_sync(mph_ctx *ctx);

@@ -66,6 +66,17 @@ class MphStats(C.Structure):
     ]
 
 
+class MphTiming(C.Structure):
+    _fields_ = [
+        ("move_ms", C.c_double),
+        ("rebuild_ms", C.c_double),
+        ("sweep_a_ms", C.c_double),
+        ("sweep_b_ms", C.c_double),
+        ("pairs", C.c_longlong),
+        ("rebuilds", C.c_longlong),
+    ]
+
+
 FIELD_SHAPES = {
     "particle_index": (np.int32, 1), "type": (np.int32, 1), "position": (np.float64, 3),
     "velocity": (np.float64, 3), "force": (np.float64, 3), "density_a": (np.float64, 1),
