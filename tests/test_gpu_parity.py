@@ -40,8 +40,32 @@ def _gpu_rows(solver):
     return [tuple(ind[ptr[i]:ptr[i + 1]]) for i in range(len(ptr) - 1)]
 
 
-def _compare(gf, of, label, names=COMPARED, tol=FP64_TOL):
-    errs = {k: normwise(gf[ORACLE_TO_ABI[k]], of[k]) for k in names}
+def _pressure_scale(deck):
+    """K_max * N0p: the pressure of a unit volumetric strain.  P = -lambda D + kappa e^P
+    (main.cpp:1905) is a multiple of the strain sum, whose rounding error is a multiple of
+    N0p, so this is the scale that FP64 differences in P are measured against when the
+    pressure field itself is at rounding level (a lattice deck at its first step)."""
+    from oracle.oracle import Oracle
+    return max(deck.data.bulk_modulus) * Oracle(deck).scalars()[1]
+
+
+def _fields_for(deck, names=COMPARED):
+    # GravityCenter is only computed when some surface tension is non-zero: without
+    # surface tension it enters no force (CofA = 0) and is not an output field
+    if not any(t != 0.0 for t in deck.data.surface_tension):
+        names = tuple(k for k in names if k != "gcenter")
+    return names
+
+
+def _compare(gf, of, label, names=COMPARED, tol=FP64_TOL, p_scale=0.0):
+    errs = {}
+    for k in names:
+        a, b = gf[ORACLE_TO_ABI[k]], of[k]
+        if k == "press_p" and np.max(np.abs(b)) < 1e-9 * p_scale:
+            # pressure field at rounding level: compare at the pressure scale K*N0p
+            errs[k] = float(np.max(np.abs(a - b)) / p_scale)
+        else:
+            errs[k] = normwise(a, b)
     print(label, " ".join("%s=%.2e" % kv for kv in errs.items()), flush=True)
     bad = {k: e for k, e in errs.items() if not e <= tol}
     assert not bad, "%s: fields beyond %g: %s" % (label, tol, bad)
@@ -77,11 +101,11 @@ def test_first_step_parity(solver_cls, name):
     ptr, ind = orc.neighbors()
     assert _gpu_rows(s) == oracle_lists_by_id(orc.fields(), ptr, ind)
     assert np.array_equal(gf["neighbor_count"], of["ncount"])
-    _compare(gf, of, name + " step 1")
+    _compare(gf, of, name + " step 1", names=_fields_for(d), p_scale=_pressure_scale(d))
     assert s.time() == orc.time
 
 
-@pytest.mark.parametrize("name", ["droplet2d", "movingwall2d", "bridge2d", "twofluid3d"])
+@pytest.mark.parametrize("name", ["droplet2d", "movingwall2d", "bridge2d", "squaredrop2d", "twofluid3d"])
 def test_multi_step_parity(solver_cls, name):
     print("case", name, flush=True)
     d = DECKS[name]()
@@ -93,7 +117,7 @@ def test_multi_step_parity(solver_cls, name):
     of = by_id(orc.fields())
     gf = s.fields()
     # rounding differences grow slowly over the steps; 1e-9 after 20 steps
-    _compare(gf, of, "%s step %d" % (name, steps), tol=1e-9)
+    _compare(gf, of, "%s step %d" % (name, steps), names=_fields_for(d), tol=1e-9)
     st = s.stats()
     assert st["steps"] == steps
 
@@ -151,8 +175,13 @@ def test_golden_fixtures(solver_cls, name):
     assert np.array_equal(f["neighbor_count"][ids], g["init_ncount"])
     s.step(1)
     f = s.fields()
+    p_scale = _pressure_scale(d)
     for k, gk in (("position", "x"), ("velocity", "v"), ("force", "force"), ("pressure_p", "press_p"),
                   ("density_a", "dens_a"), ("vol_strain_p", "vol_strain")):
-        err = normwise(f[k][ids], g["step1_" + gk])
+        ref = g["step1_" + gk]
+        if gk == "press_p" and np.max(np.abs(ref)) < 1e-9 * p_scale:
+            err = float(np.max(np.abs(f[k][ids] - ref)) / p_scale)
+        else:
+            err = normwise(f[k][ids], ref)
         assert err <= FP64_TOL, (k, err)
     assert np.array_equal(f["neighbor_count"][ids], g["step1_ncount"])
