@@ -17,8 +17,9 @@ struct CaseConst {
     int cc[3], tcc, key_base;
     int surface;            // any surface tension -> densities A/G and forces A/S needed
     int ratio_one;          // every InteractionRatio == 1
-    int pad0;
-    double dmin[3], dwid[3], hwid[3];   // domain min, width, 0.5*width
+    int rank_base;          // offset of this case in the cell rank tables
+    double dmin[3], dmax[3], dwid[3], hwid[3];   // domain min, max, width, 0.5*width
+    double guard[3];        // distance from a face beyond which no list partner wraps
     double cw[3];                       // cell widths
     double cut2;            // (MaxRadius + MARGIN)^2, list cutoff (main.cpp:1408)
     double hmax2;           // MaxRadius^2: pairs closer than this interact (pair counter)

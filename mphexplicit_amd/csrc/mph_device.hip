@@ -81,7 +81,7 @@ struct mph_ctx {
     DevBuf<int> pidx, pidx_alt, ncnt, nout, nbr, start;
     DevBuf<double> dens_a, vstrain, divp, wallc, vir;
     DevBuf<unsigned> keys_a, keys_b, hist;
-    DevBuf<int> perm_a, perm_b;
+    DevBuf<int> perm_a, perm_b, rank, cell_of_rank;
     DevBuf<int> flags, any_flag, max_len;
     DevBuf<long long> rebuilds;
     DevBuf<unsigned long long> pairs;
@@ -129,6 +129,33 @@ static double lattice_sum(int dim, double l0, double radius, double sw, bool att
                 }
             }
     return sum;
+}
+
+// Morton order of a case's cells: rank[cell id] and its inverse
+static void morton_ranks(const CaseConst& k, std::vector<int>& rank, std::vector<int>& cell_of_rank)
+{
+    auto spread = [](unsigned long long v) {
+        unsigned long long x = v & 0x1fffffull;
+        x = (x | x << 32) & 0x1f00000000ffffull;
+        x = (x | x << 16) & 0x1f0000ff0000ffull;
+        x = (x | x << 8) & 0x100f00f00f00f00full;
+        x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+        x = (x | x << 2) & 0x1249249249249249ull;
+        return x;
+    };
+    const int T = k.tcc;
+    std::vector<std::pair<unsigned long long, int>> code(T);
+    for (int ix = 0, id = 0; ix < k.cc[0]; ++ix)
+        for (int iy = 0; iy < k.cc[1]; ++iy)
+            for (int iz = 0; iz < k.cc[2]; ++iz, ++id)
+                code[id] = {spread(ix) << 2 | spread(iy) << 1 | spread(iz), id};
+    std::sort(code.begin(), code.end());
+    rank.resize(T);
+    cell_of_rank.resize(T);
+    for (int r = 0; r < T; ++r) {
+        cell_of_rank[r] = code[r].second;
+        rank[code[r].second] = r;
+    }
 }
 
 static int derive_case(const mph_params& p, int begin, int end, int nfluid, int key_base, CaseConst& k)
@@ -229,6 +256,7 @@ static int derive_case(const mph_params& p, int begin, int end, int nfluid, int 
     double cnt[3];
     for (int d = 0; d < 3; ++d) {
         k.dmin[d] = p.domain_min[d];
+        k.dmax[d] = p.domain_max[d];
         k.dwid[d] = p.domain_max[d] - p.domain_min[d];
         k.hwid[d] = 0.5 * k.dwid[d];
         if (!(k.dwid[d] > 0.0)) return fail(MPH_ERR_ARG, "domain width on axis %d must be positive", d);
@@ -246,6 +274,10 @@ static int derive_case(const mph_params& p, int begin, int end, int nfluid, int 
     k.tcc = (int)tcc;
     for (int d = 0; d < 3; ++d) k.cw[d] = k.dwid[d] / k.cc[d];
     k.cut2 = (maxr + margin) * (maxr + margin);                        // main.cpp:1408
+    // list partners are within the list cut-off at the last rebuild and each particle has
+    // moved less than margin/2 since (main.cpp:1102)
+    const double reach = maxr + 2.5 * margin;
+    for (int d = 0; d < 3; ++d) k.guard[d] = k.hwid[d] > 2.0 * reach ? reach : HUGE_VAL;
     k.hmax2 = maxr * maxr;
     k.shift2 = 0.5 * margin * 0.5 * margin;                             // main.cpp:1102
     return MPH_OK;
@@ -259,7 +291,7 @@ static int launch_rebuild(mph_ctx* c, int count_only)
     hipStream_t s = c->stream;
     unsigned* kf = c->sorted_keys();
     hipLaunchKernelGGL(k_keys, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p, c->pos.p,
-                       c->flags.p, c->any_flag.p, kf, c->keys_a.p, c->perm_a.p);
+                       c->flags.p, c->any_flag.p, c->rank.p, kf, c->keys_a.p, c->perm_a.p);
     for (int ps = 0; ps < c->sort_passes; ++ps) {
         unsigned* kin = (ps & 1) ? c->keys_b.p : c->keys_a.p;
         unsigned* kout = (ps & 1) ? c->keys_a.p : c->keys_b.p;
@@ -282,7 +314,8 @@ static int launch_rebuild(mph_ctx* c, int count_only)
                        c->flags.p, c->any_flag.p, c->pos_alt.p, c->vel_alt.p, c->pidx_alt.p, c->pos.p,
                        c->vel.p, c->pidx.p, c->snap.p);
     hipLaunchKernelGGL(k_build_lists, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p,
-                       c->flags.p, c->any_flag.p, c->pos.p, kf, c->start.p, c->nbr.p, c->stride,
+                       c->flags.p, c->any_flag.p, c->pos.p, kf, c->start.p, c->rank.p,
+                       c->cell_of_rank.p, c->nbr.p, c->stride,
                        count_only ? 0 : c->width, c->ncnt.p, c->nout.p, c->max_len.p);
     return hipGetLastError() == hipSuccess ? MPH_OK : fail(MPH_ERR_DEVICE, "rebuild launch failed");
 }
@@ -376,6 +409,8 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     std::vector<int> hidx(c->n);
     std::vector<double> hwall((size_t)ncases * kTypes * 3, 0.0);
     int begin = 0, key_base = 0;
+    std::vector<int> hrank, hcell;
+    std::vector<int2> logical;
     c->any_surface = 0;
     for (int k = 0; k < ncases; ++k) {
         const int nk = counts[k];
@@ -404,8 +439,30 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
             hvel[g] = make_double4(v[3 * g], v[3 * g + 1], v[3 * g + 2], 0.0);
             hidx[g] = i;
         }
-        for (int b = 0; b < nk; b += kBlock) c->bmap_host.push_back(make_int2(k, begin + b));
+        for (int i = 0; i < nk; ++i)
+            for (int d = 0; d < 3; ++d) {
+                const double xv = x[3 * (size_t)(begin + i) + d];
+                if (!(xv >= ch.k.dmin[d] - 0.25 * ch.k.dwid[d] && xv <= ch.k.dmax[d] + 0.25 * ch.k.dwid[d]))
+                    return fail(MPH_ERR_ARG, "case %d particle %d lies outside the periodic domain", k, i);
+            }
+        std::vector<int> rk, cr;
+        morton_ranks(ch.k, rk, cr);
+        ch.k.rank_base = (int)hrank.size();
+        hrank.insert(hrank.end(), rk.begin(), rk.end());
+        hcell.insert(hcell.end(), cr.begin(), cr.end());
+        for (int b = 0; b < nk; b += kBlock) logical.push_back(make_int2(k, begin + b));
         begin += nk;
+    }
+    // XCD-aware order: physical workgroup w runs on XCD w % kXcds; give each XCD one
+    // contiguous run of the logical (particle-ordered) blocks
+    {
+        const int nb = (int)logical.size();
+        const int per = (nb + kXcds - 1) / kXcds;
+        c->bmap_host.assign((size_t)per * kXcds, make_int2(-1, 0));
+        for (int w = 0; w < per * kXcds; ++w) {
+            const int L = (w % kXcds) * per + w / kXcds;
+            if (L < nb) c->bmap_host[w] = logical[L];
+        }
     }
     c->nkeys = key_base;
     int bits = 1;
@@ -426,7 +483,8 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         (rc = c->vstrain.alloc(n)) || (rc = c->divp.alloc(n)) || (rc = c->vir.alloc(n)) || (rc = c->wallc.alloc(hwall.size())) ||
         (rc = c->keys_a.alloc(n)) || (rc = c->keys_b.alloc(n)) ||
         (rc = c->hist.alloc((size_t)kRadix * c->sort_blocks)) || (rc = c->perm_a.alloc(n)) ||
-        (rc = c->perm_b.alloc(n)) || (rc = c->flags.alloc(ncases)) || (rc = c->any_flag.alloc(1)) ||
+        (rc = c->perm_b.alloc(n)) || (rc = c->flags.alloc(ncases)) ||
+        (rc = c->rank.alloc(hrank.size())) || (rc = c->cell_of_rank.alloc(hcell.size())) || (rc = c->any_flag.alloc(1)) ||
         (rc = c->max_len.alloc(1)) || (rc = c->rebuilds.alloc(ncases)) || (rc = c->pairs.alloc(1)))
         return rc;
     hipStream_t s = c->stream;
@@ -436,6 +494,8 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     HIP_TRY(hipMemcpyAsync(c->vel.p, hvel.data(), sizeof(double4) * n, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->pidx.p, hidx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->wallc.p, hwall.data(), sizeof(double) * hwall.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->rank.p, hrank.data(), sizeof(int) * hrank.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->cell_of_rank.p, hcell.data(), sizeof(int) * hcell.size(), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(c->aux.p, 0, sizeof(double4) * n, s));
     HIP_TRY(hipMemsetAsync(c->force.p, 0, sizeof(double4) * n, s));
     HIP_TRY(hipMemsetAsync(c->vir.p, 0, sizeof(double) * n, s));

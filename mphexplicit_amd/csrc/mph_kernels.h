@@ -12,7 +12,16 @@
 //
 // Every particle kernel is launched over a block map: block b handles particles
 // [map[b].y, min(map[b].y + blockDim, case.end)) of case map[b].x, so the case constants
-// are uniform per workgroup.
+// are uniform per workgroup.  The map is laid out for the 8 XCDs of the MI355X: the
+// hardware hands workgroup b to XCD b % 8, and the map gives each XCD one contiguous run
+// of particle blocks (entries with case -1 are padding), so a spatially compact region
+// shares one XCD's L2.
+//
+// Cells are stored in Morton order: the sort key of a particle is
+//   key_base + 2*rank(cell) + (wall ? 1 : 0)
+// so a cell's fluid particles are followed by its wall particles and consecutive particles
+// are close in space in all three directions.  The reference's own cell ids (main.cpp:96-100)
+// still define the grid, the stencil and the order in which lists are filled.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,6 +34,7 @@ constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
+constexpr int kXcds = 8;   // MI355X: 8 XCDs x 32 CUs, workgroups dispatched round-robin
 
 __device__ __forceinline__ int tag_of(double w) { return (int)__double_as_longlong(w); }
 
@@ -49,9 +59,36 @@ __device__ __forceinline__ void rel3(const CaseConst& C, const double4& pi, cons
     dz = mod_w(pj.z - pi.z + C.hwid[2], C.dwid[2]) - C.hwid[2];
 }
 
-__device__ __forceinline__ double sel4(int t, double a0, double a1, double a2, double a3)
+// The same difference for a list partner, without the division: with positions inside
+// the periodic box, a = x_j - x_i + W/2 lies in [-W, 2W), where floor(a/W) is -1, 0 or 1.
+__device__ __forceinline__ double rel_wrap(double xj, double xi, double hw, double w)
 {
-    return t == 0 ? a0 : (t == 1 ? a1 : (t == 2 ? a2 : a3));
+    const double a = xj - xi + hw;
+    const double s = a < 0.0 ? -w : (a >= w ? w : 0.0);
+    return (a - s) - hw;
+}
+
+// ... and for a particle far enough from the periodic faces that no partner wraps
+__device__ __forceinline__ double rel_in(double xj, double xi, double hw) { return (xj - xi + hw) - hw; }
+
+// 1/sqrt(q2): hardware estimate and two Newton steps (about 1 ulp)
+__device__ __forceinline__ double rsqrt_nr(double q2)
+{
+    double y = __builtin_amdgcn_rsq(q2);
+    const double h = 0.5 * q2;
+    double e = fma(-h * y, y, 0.5);
+    y = fma(y, e, y);
+    e = fma(-h * y, y, 0.5);
+    return fma(y, e, y);
+}
+
+// true if no neighbour-list partner of a particle at p can need a periodic image on any
+// axis: p is at least `guard` (list cut-off + Verlet skin) inside every face
+__device__ __forceinline__ bool away_from_faces(const CaseConst& C, const double4& p)
+{
+    return p.x - C.dmin[0] >= C.guard[0] && C.dmax[0] - p.x >= C.guard[0] &&
+           p.y - C.dmin[1] >= C.guard[1] && C.dmax[1] - p.y >= C.guard[1] &&
+           p.z - C.dmin[2] >= C.guard[2] && C.dmax[2] - p.z >= C.guard[2];
 }
 
 // -------------------------------------------------------------------------------------
@@ -65,6 +102,7 @@ k_move(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
        int* __restrict__ flags, int* __restrict__ any_flag)
 {
     const int2 m = bmap[blockIdx.x];
+    if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
     const int i = m.y + (int)threadIdx.x;
     bool exceed = false;
@@ -115,8 +153,9 @@ k_move(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
 // -------------------------------------------------------------------------------------
 // bucketing: cell key, LSD radix sort, cell starts, reorder (main.cpp:1111-1326)
 
-// CellId of main.cpp:96-100 (the 3D form; 2D cases have cc[2] == 1)
-__device__ __forceinline__ int cell_key(const CaseConst& C, const double4& p, int type)
+// CellId of main.cpp:96-100 (the 3D form; 2D cases have cc[2] == 1) -> storage key
+__device__ __forceinline__ int cell_key(const CaseConst& C, const int* __restrict__ rank,
+                                        const double4& p, int type)
 {
     int ix = ((int)floor((p.x - C.dmin[0]) / C.cw[0])) % C.cc[0];
     int iy = ((int)floor((p.y - C.dmin[1]) / C.cw[1])) % C.cc[1];
@@ -125,24 +164,26 @@ __device__ __forceinline__ int cell_key(const CaseConst& C, const double4& p, in
     iy = (iy % C.cc[1] + C.cc[1]) % C.cc[1];
     iz = (iz % C.cc[2] + C.cc[2]) % C.cc[2];
     const int id = (ix * C.cc[1] + iy) * C.cc[2] + iz;
-    return C.key_base + id + (type >= 2 ? C.tcc : 0);
+    return C.key_base + 2 * rank[C.rank_base + id] + (type >= 2 ? 1 : 0);
 }
 
 // keys of rebuilding cases from positions; other cases keep their sorted keys
 __global__ void __launch_bounds__(256)
 k_keys(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
        const double4* __restrict__ pos, const int* __restrict__ flags,
-       const int* __restrict__ any_flag, const unsigned* kprev, unsigned* kout, int* vout)
+       const int* __restrict__ any_flag, const int* __restrict__ rank, const unsigned* kprev,
+       unsigned* kout, int* vout)
 {
     if (*any_flag == 0) return;
     const int2 m = bmap[blockIdx.x];
+    if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
     const int i = m.y + (int)threadIdx.x;
     if (i >= C.end) return;
     unsigned key;
     if (flags[m.x]) {
         const double4 p = pos[i];
-        key = (unsigned)cell_key(C, p, tag_of(p.w) & 3);
+        key = (unsigned)cell_key(C, rank, p, tag_of(p.w) & 3);
     } else {
         key = kprev[i];
     }
@@ -268,6 +309,7 @@ k_gather(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
 {
     if (*any_flag == 0) return;
     const int2 m = bmap[blockIdx.x];
+    if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
     const int i = m.y + (int)threadIdx.x;
     if (i >= C.end || flags[m.x] == 0) return;
@@ -286,6 +328,7 @@ k_scatter_back(const CaseConst* __restrict__ cases, const int2* __restrict__ bma
 {
     if (*any_flag == 0) return;
     const int2 m = bmap[blockIdx.x];
+    if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
     const int i = m.y + (int)threadIdx.x;
     if (i >= C.end || flags[m.x] == 0) return;
@@ -299,16 +342,17 @@ k_scatter_back(const CaseConst* __restrict__ cases, const int2* __restrict__ bma
 // -------------------------------------------------------------------------------------
 // neighbour lists (main.cpp:1329-1596): for each particle the fluid candidates of the
 // 3^d periodic cells, then (unless it is a wall particle with no fluid neighbour,
-// main.cpp:1416) the wall candidates; kept if q^2 <= (MaxRadius+MARGIN)^2.  Cells are
-// visited in ascending id, so each list is ascending in sorted index like the
-// reference's.  Self is not stored; count_out carries the reference's NeighborCount.
+// main.cpp:1416) the wall candidates; kept if q^2 <= (MaxRadius+MARGIN)^2, with the
+// reference's exact periodic difference.  Cells are visited in ascending reference cell
+// id, as the reference fills its lists.  Self is not stored; count_out carries the
+// reference's NeighborCount (list length including self).
 
 __device__ __forceinline__ int axis_window(int ic, int c, int range, int* out)
 {
-    // the periodic window {ic-1, ic, ic+1} (or {ic} for range 0) in ascending order
+    // the periodic window {ic-1, ic, ic+1} (or {ic} for range 0) in ascending order,
+    // repeats included when the axis has fewer than 3 cells (as main.cpp:1365-1391)
     if (range == 0) { out[0] = ic; return 1; }
     int a = (ic - 1 + c) % c, b = ic, d = (ic + 1) % c;
-    // sort three values
     if (a > b) { int t = a; a = b; b = t; }
     if (b > d) { int t = b; b = d; d = t; }
     if (a > b) { int t = a; a = b; b = t; }
@@ -320,20 +364,22 @@ __global__ void __launch_bounds__(256)
 k_build_lists(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
               const int* __restrict__ flags, const int* __restrict__ any_flag,
               const double4* __restrict__ pos, const unsigned* __restrict__ keys,
-              const int* __restrict__ start, int* __restrict__ nbr, int stride, int width,
+              const int* __restrict__ start, const int* __restrict__ rank,
+              const int* __restrict__ cell_of_rank, int* __restrict__ nbr, int stride, int width,
               int* __restrict__ ncnt, int* __restrict__ count_out, int* __restrict__ max_len)
 {
     if (*any_flag == 0) return;
     const int2 m = bmap[blockIdx.x];
+    if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
     if (flags[m.x] == 0) return;          // uniform: a workgroup belongs to one case
     const int i = m.y + (int)threadIdx.x;
     int count = 0;                        // stored entries (self excluded)
     if (i < C.end) {
         const double4 pi = pos[i];
-        int cid = (int)keys[i] - C.key_base;
-        const bool wall = cid >= C.tcc;
-        if (wall) cid -= C.tcc;
+        const int kk = (int)keys[i] - C.key_base;
+        const bool wall = kk & 1;
+        const int cid = cell_of_rank[C.rank_base + (kk >> 1)];
         const int iz = cid % C.cc[2];
         const int iy = (cid / C.cc[2]) % C.cc[1];
         const int ix = cid / (C.cc[1] * C.cc[2]);
@@ -344,12 +390,12 @@ k_build_lists(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap
         int seen_self = 0;
         for (int pass = 0; pass < 2; ++pass) {
             if (pass == 1 && wall && count == 0) break;
-            const int off = C.key_base + (pass ? C.tcc : 0);
             for (int a = 0; a < nx; ++a)
                 for (int b = 0; b < ny; ++b)
                     for (int c = 0; c < nz; ++c) {
                         const int cell = (wx[a] * C.cc[1] + wy[b]) * C.cc[2] + wz[c];
-                        const int lo = start[off + cell], hi = start[off + cell + 1];
+                        const int k0 = C.key_base + 2 * rank[C.rank_base + cell] + pass;
+                        const int lo = start[k0], hi = start[k0 + 1];
                         for (int j = lo; j < hi; ++j) {
                             const double4 pj = pos[j];
                             double dx, dy, dz;
@@ -375,6 +421,74 @@ k_build_lists(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap
 // sweep A: number densities, gravity centre, volumetric strain, divergence
 // (main.cpp:1665-1727, 1831-1896) with the pointwise coefficients and pressures of
 // main.cpp:1631-1662, 1898-1906, 1729-1740 fused into the epilogue.
+//
+// Per pair: exact periodic difference (the division-free forms above), q^2, one
+// reciprocal square root, then the kernel polynomials with their constant prefactors
+// pulled out of the sums.  The loop is software pipelined: the list index two entries
+// ahead and the particle data one entry ahead are in flight while a pair is evaluated.
+
+struct PairA {
+    double4 p;
+    double vx, vy, vz;
+};
+
+__device__ __forceinline__ PairA load_a(const double4* __restrict__ pos, const double4* vel, int j)
+{
+    PairA q;
+    q.p = pos[j];
+    q.vx = vel[j].x;          // P (vel.w) of partners is being written by this kernel
+    q.vy = vel[j].y;
+    q.vz = vel[j].z;
+    return q;
+}
+
+template <int SURF, bool WRAP>
+__device__ __forceinline__ void sweep_a_loop(const CaseConst& C, const double* tab, int ti,
+                                             const double4& pi, double vix, double viy, double viz,
+                                             const double4* __restrict__ pos, const double4* vel,
+                                             const int* __restrict__ nbr, int stride, int i, int cnt,
+                                             double& na, double& gx, double& gy, double& gz,
+                                             double& wsum, double& dsum, int& npair)
+{
+    int jn = cnt > 1 ? nbr[stride + i] : i;
+    PairA nx = load_a(pos, vel, cnt > 0 ? nbr[i] : i);
+    for (int k = 0; k < cnt; ++k) {
+        const PairA q = nx;
+        if (k + 1 < cnt) nx = load_a(pos, vel, jn);
+        if (k + 2 < cnt) jn = nbr[(size_t)(k + 2) * stride + i];
+        const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(q.p.x, pi.x, C.hwid[0]);
+        const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(q.p.y, pi.y, C.hwid[1]);
+        const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(q.p.z, pi.z, C.hwid[2]);
+        const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        npair += q2 <= C.hmax2;
+        const bool in_a = C.ha2 - q2 >= 0.0;
+        const bool in_p = C.hp2 - q2 >= 0.0;
+        if (in_a || in_p) {
+            const double ir = rsqrt_nr(q2);
+            const double r = q2 * ir;
+            if (in_a) {
+                const double ratio = C.ratio_one ? 1.0 : tab[16 + ti * 4 + (tag_of(q.p.w) & 3)];
+                const double s = r * C.inv_ha;
+                const double om = 1.0 - s;
+                const double om2 = om * om;
+                na = fma(ratio * s, om2, na);
+                if (SURF) {
+                    const double wg = ratio * om2;
+                    gx = fma(dx, wg, gx);
+                    gy = fma(dy, wg, gy);
+                    gz = fma(dz, wg, gz);
+                }
+            }
+            if (in_p) {
+                const double om = fma(-r, C.inv_hp, 1.0);
+                wsum = fma(om, om, wsum);
+                const double ux = q.vx - vix, uy = q.vy - viy, uz = q.vz - viz;
+                const double ud = fma(uz, dz, fma(uy, dy, ux * dx));
+                dsum = fma(ud, om * ir, dsum);
+            }
+        }
+    }
+}
 
 template <int SURF>
 __global__ void __launch_bounds__(256)
@@ -384,8 +498,6 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
           const int* __restrict__ nbr, const int* __restrict__ ncnt, int stride,
           double* __restrict__ wallc, int ncases, unsigned long long* __restrict__ pair_count)
 {
-    const int2 m = bmap[blockIdx.x];
-    const CaseConst& C = cases[m.x];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // wall centres advance once per step (main.cpp:2048-2052); the move kernel of
         // this step has already used the old value
@@ -394,56 +506,42 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
                 for (int d = 0; d < 3; ++d)
                     wallc[(c * kTypes + t) * 3 + d] += cases[c].wvel[t][d] * cases[c].dt;
     }
+    const int2 m = bmap[blockIdx.x];
+    if (m.x < 0) return;
+    const CaseConst& C = cases[m.x];
+    __shared__ double tab[32];                // [0,16): mu_ij, [16,32): InteractionRatio
+    if (threadIdx.x < 16) {
+        tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
+        tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
+    }
+    __syncthreads();
     const int i = m.y + (int)threadIdx.x;
     int npair = 0;
     if (i < C.end) {
         const double4 pi = pos[i];
         const int ti = tag_of(pi.w) & 3;
         const double vix = vel[i].x, viy = vel[i].y, viz = vel[i].z;
-        const double r0 = C.ir[ti][0], r1 = C.ir[ti][1], r2 = C.ir[ti][2], r3 = C.ir[ti][3];
         double na = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, wsum = 0.0, dsum = 0.0;
         const int cnt = ncnt[i];
-        for (int k = 0; k < cnt; ++k) {
-            const int j = nbr[(size_t)k * stride + i];
-            const double4 pj = pos[j];
-            double dx, dy, dz;
-            rel3(C, pi, pj, dx, dy, dz);
-            const double q2 = dx * dx + dy * dy + dz * dz;
-            npair += q2 <= C.hmax2;
-            const bool in_a = C.ha2 - q2 >= 0.0;
-            const bool in_p = C.hp2 - q2 >= 0.0;
-            if (!(in_a || in_p)) continue;
-            const double r = sqrt(q2);
-            if (in_a) {
-                const double ratio = C.ratio_one ? 1.0 : sel4(tag_of(pj.w) & 3, r0, r1, r2, r3);
-                const double s = r * C.inv_ha;
-                const double om = 1.0 - s;
-                na += ratio * (C.ca * s * om * om);
-                if (SURF) {
-                    const double wg = ratio * (C.cg * om * om);
-                    gx += dx * wg;
-                    gy += dy * wg;
-                    gz += dz * wg;
-                }
-            }
-            if (in_p) {
-                const double s = r * C.inv_hp;
-                const double om = 1.0 - s;
-                wsum += C.cp * om * om;
-                const double ux = vel[j].x - vix, uy = vel[j].y - viy, uz = vel[j].z - viz;
-                dsum -= (ux * dx + uy * dy + uz * dz) * (C.dcp * om / r);
-            }
-        }
-        const double ep = wsum - C.n0p;
+        if (away_from_faces(C, pi))
+            sweep_a_loop<SURF, false>(C, tab, ti, pi, vix, viy, viz, pos, vel, nbr, stride, i, cnt,
+                                      na, gx, gy, gz, wsum, dsum, npair);
+        else
+            sweep_a_loop<SURF, true>(C, tab, ti, pi, vix, viy, viz, pos, vel, nbr, stride, i, cnt,
+                                     na, gx, gy, gz, wsum, dsum, npair);
+        na *= C.ca;
+        const double ep = C.cp * wsum - C.n0p;
+        const double dv = -C.dcp * dsum;
         const double kap = ep < 0.0 ? 0.0 : C.kbulk[ti];
-        const double p = -C.lbulk[ti] * dsum + kap * ep;
+        const double p = -C.lbulk[ti] * dv + kap * ep;
         double pa = C.cofa[ti] * (na - C.n0a) / C.l0;
         if (C.n0a <= na) pa = 0.0;
+        const double gs = C.cg * C.g_scale;
         vel[i].w = p;
-        aux[i] = make_double4(gx * C.g_scale, gy * C.g_scale, gz * C.g_scale, pa);
+        aux[i] = make_double4(gx * gs, gy * gs, gz * gs, pa);
         dens_a[i] = na;
         vstrain[i] = ep;
-        divp[i] = dsum;
+        divp[i] = dv;
     }
     for (int off = 32; off > 0; off >>= 1) npair += __shfl_xor(npair, off);
     if ((threadIdx.x & 63) == 0 && npair) atomicAdd(pair_count, (unsigned long long)npair);
@@ -452,7 +550,84 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
 // -------------------------------------------------------------------------------------
 // sweep B: pressure, attractive-pressure, diffuse-interface and viscous forces
 // (main.cpp:1908-1938, 1742-1775, 1778-1829, 1941-1985), gravity and the velocity update
-// (main.cpp:1988-2010).  Force is summed per term and combined in the reference's order.
+// (main.cpp:1988-2010).  Each force term is summed separately with its constant factors
+// pulled out, and the terms are combined in the reference's order.
+
+struct PairB {
+    double4 p, v, a;
+};
+
+template <int SURF>
+__device__ __forceinline__ PairB load_b(const double4* __restrict__ pos, const double4* __restrict__ vel,
+                                        const double4* __restrict__ aux, int j)
+{
+    PairB q;
+    q.p = pos[j];
+    q.v = vel[j];
+    if (SURF) q.a = aux[j];
+    return q;
+}
+
+template <int SURF, bool WRAP>
+__device__ __forceinline__ void sweep_b_loop(const CaseConst& C, const double* tab, int ti,
+                                             const double4& pi, const double4& vi, const double4& ai,
+                                             const double4* __restrict__ pos, const double4* __restrict__ vel,
+                                             const double4* __restrict__ aux, const int* __restrict__ nbr,
+                                             int stride, int i, int cnt, double3& fp, double3& fv,
+                                             double3& fa, double3& fs)
+{
+    int jn = cnt > 1 ? nbr[stride + i] : i;
+    PairB nx = load_b<SURF>(pos, vel, aux, cnt > 0 ? nbr[i] : i);
+    for (int k = 0; k < cnt; ++k) {
+        const PairB q = nx;
+        if (k + 1 < cnt) nx = load_b<SURF>(pos, vel, aux, jn);
+        if (k + 2 < cnt) jn = nbr[(size_t)(k + 2) * stride + i];
+        const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(q.p.x, pi.x, C.hwid[0]);
+        const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(q.p.y, pi.y, C.hwid[1]);
+        const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(q.p.z, pi.z, C.hwid[2]);
+        const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const bool in_p = C.hp2 - q2 > 0.0;
+        const bool in_v = C.hv2 - q2 > 0.0;
+        const bool in_a = SURF && (C.ha2 - q2 > 0.0);
+        if (!(in_p || in_v || in_a)) continue;
+        const double ir = rsqrt_nr(q2);
+        const double r = q2 * ir;
+        const int tj = tag_of(q.p.w) & 3;
+        if (in_p) {
+            const double c = (vi.w + q.v.w) * fma(-r, C.inv_hp, 1.0) * ir;
+            fp.x = fma(c, dx, fp.x);
+            fp.y = fma(c, dy, fp.y);
+            fp.z = fma(c, dz, fp.z);
+        }
+        if (in_v) {
+            const double ud = fma(q.v.z - vi.z, dz, fma(q.v.y - vi.y, dy, (q.v.x - vi.x) * dx));
+            const double c = tab[ti * 4 + tj] * ud * fma(-r, C.inv_hv, 1.0) * (ir * ir * ir);
+            fv.x = fma(c, dx, fv.x);
+            fv.y = fma(c, dy, fv.y);
+            fv.z = fma(c, dz, fv.z);
+        }
+        if (SURF && in_a) {
+            const double rij = C.ratio_one ? 1.0 : tab[16 + ti * 4 + tj];
+            const double rji = C.ratio_one ? 1.0 : tab[16 + tj * 4 + ti];
+            const double s = r * C.inv_ha;
+            const double om = 1.0 - s;
+            const double c = fma(q.a.w, rji, ai.w * rij) * om * fma(-3.0, s, 1.0) * ir;
+            fa.x = fma(c, dx, fa.x);
+            fa.y = fma(c, dy, fa.y);
+            fa.z = fma(c, dz, fa.z);
+            // diffuse interface with Q = R_ji G_j - R_ij G_i; a_j uses the type of i
+            // (main.cpp:1792), so a_i factors out of the sum
+            const double qx = fma(rji, q.a.x, -rij * ai.x);
+            const double qy = fma(rji, q.a.y, -rij * ai.y);
+            const double qz = fma(rji, q.a.z, -rij * ai.z);
+            const double qd = fma(qz, dz, fma(qy, dy, qx * dx)) * (C.dcg * ir);
+            const double w = C.cg * om;
+            fs.x = fma(-om, fma(qd, dx, w * qx), fs.x);
+            fs.y = fma(-om, fma(qd, dy, w * qy), fs.y);
+            fs.z = fma(-om, fma(qd, dz, w * qz), fs.z);
+        }
+    }
+}
 
 template <int SURF>
 __global__ void __launch_bounds__(256)
@@ -463,8 +638,6 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
           const int* __restrict__ ncnt, int stride, int* __restrict__ flags,
           int* __restrict__ any_flag, long long* __restrict__ rebuilds, int ncases)
 {
-    const int2 m = bmap[blockIdx.x];
-    const CaseConst& C = cases[m.x];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (int c = 0; c < ncases; ++c) {
             if (flags[c]) rebuilds[c] += 1;
@@ -472,86 +645,37 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
         }
         *any_flag = 0;
     }
+    const int2 m = bmap[blockIdx.x];
+    if (m.x < 0) return;
+    const CaseConst& C = cases[m.x];
+    __shared__ double tab[32];
+    if (threadIdx.x < 16) {
+        tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
+        tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
+    }
+    __syncthreads();
     const int i = m.y + (int)threadIdx.x;
     if (i >= C.end) return;
     const double4 pi = pos[i];
     const double4 vi = vel[i];
     const int ti = tag_of(pi.w) & 3;
-    const double mu0 = C.muij[ti][0], mu1 = C.muij[ti][1], mu2 = C.muij[ti][2], mu3 = C.muij[ti][3];
-    double fpx = 0.0, fpy = 0.0, fpz = 0.0;
-    double fvx = 0.0, fvy = 0.0, fvz = 0.0;
-    double fax = 0.0, fay = 0.0, faz = 0.0;
-    double fsx = 0.0, fsy = 0.0, fsz = 0.0;
-    double4 ai = make_double4(0.0, 0.0, 0.0, 0.0);
-    double ri0 = 1.0, ri1 = 1.0, ri2 = 1.0, ri3 = 1.0;
-    if (SURF) {
-        ai = aux[i];
-        ri0 = C.ir[ti][0]; ri1 = C.ir[ti][1]; ri2 = C.ir[ti][2]; ri3 = C.ir[ti][3];
-    }
+    const double4 ai = SURF ? aux[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    double3 fp = make_double3(0.0, 0.0, 0.0), fv = fp, fa = fp, fs = fp;
     const int cnt = ncnt[i];
-    for (int k = 0; k < cnt; ++k) {
-        const int j = nbr[(size_t)k * stride + i];
-        const double4 pj = pos[j];
-        double dx, dy, dz;
-        rel3(C, pi, pj, dx, dy, dz);
-        const double q2 = dx * dx + dy * dy + dz * dz;
-        const bool in_p = C.hp2 - q2 > 0.0;
-        const bool in_v = C.hv2 - q2 > 0.0;
-        const bool in_a = SURF && (C.ha2 - q2 > 0.0);
-        if (!(in_p || in_v || in_a)) continue;
-        const double r = sqrt(q2);
-        const double ir = 1.0 / r;
-        const double4 vj = vel[j];
-        const int tj = tag_of(pj.w) & 3;
-        if (in_p) {
-            const double om = 1.0 - r * C.inv_hp;
-            const double c = (vi.w + vj.w) * (C.dcp * om) * ir;
-            fpx += c * dx;
-            fpy += c * dy;
-            fpz += c * dz;
-        }
-        if (in_v) {
-            const double om = 1.0 - r * C.inv_hv;
-            const double muij = sel4(tj, mu0, mu1, mu2, mu3);
-            const double ue = ((vj.x - vi.x) * dx + (vj.y - vi.y) * dy + (vj.z - vi.z) * dz) * ir;
-            const double c = C.visc_c * muij * ue * (-(C.dcv * om)) * ir * ir;
-            fvx += c * dx;
-            fvy += c * dy;
-            fvz += c * dz;
-        }
-        if (SURF && in_a) {
-            const double4 aj = aux[j];
-            const double rij = C.ratio_one ? 1.0 : sel4(tj, ri0, ri1, ri2, ri3);
-            const double rji = C.ratio_one ? 1.0 : C.ir[tj][ti];
-            const double s = r * C.inv_ha;
-            const double om = 1.0 - s;
-            const double dwa = C.dca * om * (1.0 - 3.0 * s);
-            const double ca = (ai.w * rij + aj.w * rji) * dwa * ir;
-            fax += ca * dx;
-            fay += ca * dy;
-            faz += ca * dz;
-            // diffuse interface; a_j uses the type of i (main.cpp:1792)
-            const double wg = C.cg * om * om;
-            const double dwg = C.dcg * om;
-            const double wij = rij * wg, wji = rji * wg;
-            const double dwij = rij * dwg, dwji = rji * dwg;
-            const double gr = ((aj.x * dwji - ai.x * dwij) * dx + (aj.y * dwji - ai.y * dwij) * dy +
-                               (aj.z * dwji - ai.z * dwij) * dz) * ir;
-            // gr = (G_j dw_ji - G_i dw_ij) . x_ij / r, times x_ij gives gr_ref * e_ij
-            // (main.cpp:1813-1821)
-            fsx -= (aj.x * wji - ai.x * wij) + gr * dx;
-            fsy -= (aj.y * wji - ai.y * wij) + gr * dy;
-            fsz -= (aj.z * wji - ai.z * wij) + gr * dz;
-        }
-    }
-    const double V0 = C.vol;
-    double fx = fpx * V0, fy = fpy * V0, fz = fpz * V0;
+    if (away_from_faces(C, pi))
+        sweep_b_loop<SURF, false>(C, tab, ti, pi, vi, ai, pos, vel, aux, nbr, stride, i, cnt, fp, fv, fa, fs);
+    else
+        sweep_b_loop<SURF, true>(C, tab, ti, pi, vi, ai, pos, vel, aux, nbr, stride, i, cnt, fp, fv, fa, fs);
+    const double cp = C.dcp * C.vol;                      // pressure
+    const double cv = C.visc_c * (-C.dcv) * C.vol;        // viscosity
+    double fx = fp.x * cp, fy = fp.y * cp, fz = fp.z * cp;
     if (SURF) {
-        const double sa = C.adiff[ti] * C.s_scale;
-        fx += fax * V0; fy += fay * V0; fz += faz * V0;
-        fx += fsx * sa; fy += fsy * sa; fz += fsz * sa;
+        const double ca = C.dca * C.vol;
+        const double cs = C.adiff[ti] * C.s_scale;
+        fx += fa.x * ca; fy += fa.y * ca; fz += fa.z * ca;
+        fx += fs.x * cs; fy += fs.y * cs; fz += fs.z * cs;
     }
-    fx += fvx * V0; fy += fvy * V0; fz += fvz * V0;
+    fx += fv.x * cv; fy += fv.y * cv; fz += fv.z * cv;
     double4 vn = make_double4(vi.x, vi.y, vi.z, 0.0);
     if (ti < 2) {
         const double ms = C.mass[ti];
@@ -584,6 +708,7 @@ k_virial(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
          double* __restrict__ vir)
 {
     const int2 m = bmap[blockIdx.x];
+    if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
     const int i = m.y + (int)threadIdx.x;
     if (i >= C.end) return;
