@@ -133,7 +133,8 @@ def main():
         "data": "synthetic (seeded lattice deck, numpy PCG64)",
         "config": {"workload": deck.name, "particles": deck.n, "fluid": int((deck.type < 2).sum()),
                    "precision": args.precision, "dt": deck.data.dt, "graph": bool(args.graph),
-                   "list_width": st["list_width"]},
+                   "list_width": st["list_width"],
+                   "sweep_min_waves": [st["sweep_a_min_waves"], st["sweep_b_min_waves"]]},
         "pair_interactions_per_s": pairs_per_step * steps_per_s,
         "pairs_per_step": pairs_per_step,
         "simulated_seconds_per_s": deck.data.dt * steps_per_s,

@@ -96,6 +96,8 @@ typedef struct mph_stats {
     long long list_entries;           /* stored neighbour entries at the last rebuild (no self) */
     int list_width;                   /* reserved entries per particle */
     int max_list_len;                 /* longest list (without self) since create */
+    int sweep_a_min_waves;            /* occupancy variant of the density sweep picked by the autotuner */
+    int sweep_b_min_waves;            /* ... and of the force sweep */
 } mph_stats;
 
 /* Per-phase device time of mph_step_timed, summed over its steps. */

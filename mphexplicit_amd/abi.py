@@ -63,6 +63,8 @@ class MphStats(C.Structure):
         ("list_entries", C.c_longlong),
         ("list_width", C.c_int),
         ("max_list_len", C.c_int),
+        ("sweep_a_min_waves", C.c_int),
+        ("sweep_b_min_waves", C.c_int),
     ]
 
 

@@ -18,6 +18,8 @@ struct CaseConst {
     int surface;            // any surface tension -> densities A/G and forces A/S needed
     int ratio_one;          // every InteractionRatio == 1
     int rank_base;          // offset of this case in the cell rank tables
+    int mu_uniform;         // every type has the same shear viscosity
+    int pad0;
     double dmin[3], dmax[3], dwid[3], hwid[3];   // domain min, max, width, 0.5*width
     double guard[3];        // distance from a face beyond which no list partner wraps
     double cw[3];                       // cell widths

@@ -10,7 +10,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <map>
 #include <memory>
 #include <vector>
@@ -84,7 +86,11 @@ struct mph_ctx {
     DevBuf<int> perm_a, perm_b, rank, cell_of_rank;
     DevBuf<int> flags, any_flag, max_len;
     DevBuf<long long> rebuilds;
-    DevBuf<unsigned long long> pairs;
+    DevBuf<unsigned long long> pairs, pairs_scratch;
+    DevBuf<double4> tune_v, tune_f;      // sweep B outputs while the autotuner times variants
+    int variant_a = 1, variant_b = 1;    // indices into the sweep kernel variant tables
+    double4* vnext_out(int real) { return real ? vnext.p : tune_v.p; }
+    double4* force_out(int real) { return real ? force.p : tune_f.p; }
     int* h_scalar = nullptr;   // pinned host word for small readbacks
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_src = nullptr;
@@ -222,6 +228,9 @@ static int derive_case(const mph_params& p, int begin, int end, int nfluid, int 
     for (int t = 0; t < kTypes; ++t)
         for (int u = 0; u < kTypes; ++u)
             k.muij[t][u] = 2.0 * (k.mu[t] * k.mu[u]) / (k.mu[t] + k.mu[u]);
+    k.mu_uniform = 1;
+    for (int t = 1; t < kTypes; ++t)
+        if (k.mu[t] != k.mu[0]) k.mu_uniform = 0;
     for (int d = 0; d < 3; ++d) k.grav[d] = p.gravity[d];
     // rigid wall motion per dt, main.cpp:994-1033
     for (int w = 2; w < kTypes; ++w) {
@@ -320,18 +329,29 @@ static int launch_rebuild(mph_ctx* c, int count_only)
     return hipGetLastError() == hipSuccess ? MPH_OK : fail(MPH_ERR_DEVICE, "rebuild launch failed");
 }
 
-static int launch_sweep_a(mph_ctx* c, int advance_walls)
+// sweep kernel variants: minimum waves per SIMD the register allocation must allow
+// (the autotuner picks one per context by timing them on the case itself)
+constexpr int kSweepVariants = 4;
+static const int kSweepMinWaves[kSweepVariants] = {4, 5, 6, 8};
+using SweepAFn = decltype(&k_sweep_a<0, 4>);
+using SweepBFn = decltype(&k_sweep_b<0, 4>);
+static const SweepAFn kSweepA[2][kSweepVariants] = {
+    {k_sweep_a<0, 4>, k_sweep_a<0, 5>, k_sweep_a<0, 6>, k_sweep_a<0, 8>},
+    {k_sweep_a<1, 4>, k_sweep_a<1, 5>, k_sweep_a<1, 6>, k_sweep_a<1, 8>}};
+static const SweepBFn kSweepB[2][kSweepVariants] = {
+    {k_sweep_b<0, 4>, k_sweep_b<0, 5>, k_sweep_b<0, 6>, k_sweep_b<0, 8>},
+    {k_sweep_b<1, 4>, k_sweep_b<1, 5>, k_sweep_b<1, 6>, k_sweep_b<1, 8>}};
+
+// advance_walls: this launch is a real step (advances wall centres, counts pairs); the
+// autotuner launches with 0 so that repeated launches leave the state unchanged
+static int launch_sweep_a(mph_ctx* c, int advance_walls, int variant = -1)
 {
-    const int nb = c->nblocks();
+    const int v = variant >= 0 ? variant : c->variant_a;
     const int ncw = advance_walls ? c->ncases : 0;
-    if (c->any_surface)
-        hipLaunchKernelGGL(k_sweep_a<1>, dim3(nb), dim3(kBlock), 0, c->stream, c->d_cases.p, c->d_bmap.p,
-                           c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p, c->nbr.p,
-                           c->ncnt.p, c->stride, c->wallc.p, ncw, c->pairs.p);
-    else
-        hipLaunchKernelGGL(k_sweep_a<0>, dim3(nb), dim3(kBlock), 0, c->stream, c->d_cases.p, c->d_bmap.p,
-                           c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p, c->nbr.p,
-                           c->ncnt.p, c->stride, c->wallc.p, ncw, c->pairs.p);
+    hipLaunchKernelGGL(kSweepA[c->any_surface ? 1 : 0][v], dim3(c->nblocks()), dim3(kBlock), 0, c->stream,
+                       c->d_cases.p, c->d_bmap.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p,
+                       c->divp.p, c->nbr.p, c->ncnt.p, c->stride, c->wallc.p, ncw,
+                       advance_walls ? c->pairs.p : c->pairs_scratch.p);
     return hipGetLastError() == hipSuccess ? MPH_OK : fail(MPH_ERR_DEVICE, "sweep A launch failed");
 }
 
@@ -342,18 +362,13 @@ static int launch_move(mph_ctx* c)
     return hipGetLastError() == hipSuccess ? MPH_OK : fail(MPH_ERR_DEVICE, "move launch failed");
 }
 
-static int launch_sweep_b(mph_ctx* c)
+static int launch_sweep_b(mph_ctx* c, int bookkeeping = 1, int variant = -1)
 {
-    const int nb = c->nblocks();
-    hipStream_t s = c->stream;
-    if (c->any_surface)
-        hipLaunchKernelGGL(k_sweep_b<1>, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p, c->pos.p,
-                           c->vel.p, c->aux.p, c->vnext.p, c->force.p, c->nbr.p, c->ncnt.p, c->stride,
-                           c->flags.p, c->any_flag.p, c->rebuilds.p, c->ncases);
-    else
-        hipLaunchKernelGGL(k_sweep_b<0>, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p, c->pos.p,
-                           c->vel.p, c->aux.p, c->vnext.p, c->force.p, c->nbr.p, c->ncnt.p, c->stride,
-                           c->flags.p, c->any_flag.p, c->rebuilds.p, c->ncases);
+    const int v = variant >= 0 ? variant : c->variant_b;
+    hipLaunchKernelGGL(kSweepB[c->any_surface ? 1 : 0][v], dim3(c->nblocks()), dim3(kBlock), 0, c->stream,
+                       c->d_cases.p, c->d_bmap.p, c->pos.p, c->vel.p, c->aux.p, c->vnext_out(bookkeeping),
+                       c->force_out(bookkeeping), c->nbr.p, c->ncnt.p, c->stride, c->flags.p, c->any_flag.p,
+                       c->rebuilds.p, bookkeeping ? c->ncases : 0);
     return hipGetLastError() == hipSuccess ? MPH_OK : fail(MPH_ERR_DEVICE, "sweep B launch failed");
 }
 
@@ -382,6 +397,90 @@ static int check_capacity(mph_ctx* c)
     if (mx > c->width)
         return fail(MPH_ERR_CAPACITY, "a neighbour list reached %d entries, above the reserved width %d; "
                     "results of this call are invalid", mx, c->width);
+    return MPH_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// autotuner: time every sweep variant on the context's own lists and keep the fastest.
+// Results are cached in a text file (MPH_TUNE_CACHE, default ~/.cache/mphexplicit_amd/
+// sweep_variants.txt) keyed by device, dimension, surface tension and problem size;
+// MPH_TUNE=0 keeps the built-in defaults, MPH_TUNE=force ignores the cache.
+
+static std::string tune_key(mph_ctx* c)
+{
+    hipDeviceProp_t prop;
+    std::string arch = "gpu";
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) arch = prop.gcnArchName;
+    int dim = 3;
+    for (auto& ch : c->cases) dim = std::min(dim, ch.k.dim);
+    int nb = 0;
+    while ((1ll << (nb + 1)) <= (long long)c->n) ++nb;
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s|dim%d|surf%d|n2^%d|w%d", arch.c_str(), dim, c->any_surface, nb, c->width / 16);
+    return buf;
+}
+
+static std::string tune_path()
+{
+    const char* p = std::getenv("MPH_TUNE_CACHE");
+    if (p && *p) return p;
+    const char* h = std::getenv("HOME");
+    return std::string(h ? h : "/tmp") + "/.cache/mphexplicit_amd/sweep_variants.txt";
+}
+
+static int autotune(mph_ctx* c)
+{
+    const char* mode = std::getenv("MPH_TUNE");
+    if (mode && !std::strcmp(mode, "0")) return MPH_OK;
+    const std::string key = tune_key(c), path = tune_path();
+    if (!(mode && !std::strcmp(mode, "force"))) {
+        if (FILE* f = std::fopen(path.c_str(), "r")) {
+            char k[256];
+            int a, b;
+            while (std::fscanf(f, "%255s %d %d", k, &a, &b) == 3)
+                if (key == k && a >= 0 && a < kSweepVariants && b >= 0 && b < kSweepVariants) {
+                    c->variant_a = a;
+                    c->variant_b = b;
+                    std::fclose(f);
+                    return MPH_OK;
+                }
+            std::fclose(f);
+        }
+    }
+    int rc = 0;
+    if ((rc = c->tune_v.alloc(c->n)) || (rc = c->tune_f.alloc(c->n))) return rc;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best_a = 1e30f, best_b = 1e30f;
+    for (int v = 0; v < kSweepVariants && rc == MPH_OK; ++v) {
+        for (int kind = 0; kind < 2 && rc == MPH_OK; ++kind) {
+            float total = 0.0f;
+            for (int rep = 0; rep < 4 && rc == MPH_OK; ++rep) {
+                (void)hipEventRecord(e0, c->stream);
+                rc = kind == 0 ? launch_sweep_a(c, 0, v) : launch_sweep_b(c, 0, v);
+                (void)hipEventRecord(e1, c->stream);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (rep > 0) total += ms;           // the first launch warms caches
+            }
+            if (kind == 0 && total < best_a) { best_a = total; c->variant_a = v; }
+            if (kind == 1 && total < best_b) { best_b = total; c->variant_b = v; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->tune_v.release();
+    c->tune_f.release();
+    if (rc) return rc;
+    const std::string dir = path.substr(0, path.rfind('/'));
+    std::string mk = "mkdir -p '" + dir + "' 2>/dev/null";
+    (void)!std::system(mk.c_str());
+    if (FILE* f = std::fopen(path.c_str(), "a")) {
+        std::fprintf(f, "%s %d %d\n", key.c_str(), c->variant_a, c->variant_b);
+        std::fclose(f);
+    }
     return MPH_OK;
 }
 
@@ -485,7 +584,8 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         (rc = c->hist.alloc((size_t)kRadix * c->sort_blocks)) || (rc = c->perm_a.alloc(n)) ||
         (rc = c->perm_b.alloc(n)) || (rc = c->flags.alloc(ncases)) ||
         (rc = c->rank.alloc(hrank.size())) || (rc = c->cell_of_rank.alloc(hcell.size())) || (rc = c->any_flag.alloc(1)) ||
-        (rc = c->max_len.alloc(1)) || (rc = c->rebuilds.alloc(ncases)) || (rc = c->pairs.alloc(1)))
+        (rc = c->max_len.alloc(1)) || (rc = c->rebuilds.alloc(ncases)) || (rc = c->pairs.alloc(1)) ||
+        (rc = c->pairs_scratch.alloc(1)))
         return rc;
     hipStream_t s = c->stream;
     HIP_TRY(hipMemcpyAsync(c->d_cases.p, hk.data(), sizeof(CaseConst) * ncases, hipMemcpyHostToDevice, s));
@@ -525,7 +625,8 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     std::vector<long long> one_ll(ncases, 1);
     HIP_TRY(hipMemcpyAsync(c->rebuilds.p, one_ll.data(), sizeof(long long) * ncases, hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
-    return check_capacity(c);
+    if ((rc = check_capacity(c))) return rc;
+    return autotune(c);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -722,8 +823,9 @@ extern "C" int mph_get_neighbors(mph_ctx* c, int case_id, long long* ptr, int* i
     if ((rc = fetch(c, c->pidx.p, b, m, idx)) || (rc = fetch(c, c->ncnt.p, b, m, cnt)) ||
         (rc = fetch(c, c->nout.p, b, m, nc)))
         return rc;
-    HIP_TRY(hipMemcpy2DAsync(ell.data(), sizeof(int) * m, c->nbr.p + b, sizeof(int) * c->stride,
-                             sizeof(int) * m, c->width, hipMemcpyDeviceToHost, c->stream));
+    // groups of four entries: rows of the 2D copy are groups, columns the case's particles
+    HIP_TRY(hipMemcpy2DAsync(ell.data(), sizeof(int) * 4 * m, c->nbr.p + (size_t)4 * b, sizeof(int) * 4 * c->stride,
+                             sizeof(int) * 4 * m, c->width / 4, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     // rows by original id: reference lists include self (when the particle has a list)
     std::vector<long long> rowlen(m, 0);
@@ -736,7 +838,7 @@ extern "C" int mph_get_neighbors(mph_ctx* c, int case_id, long long* ptr, int* i
         long long w = ptr[o];
         std::vector<int> row;
         for (int e = 0; e < cnt[i]; ++e) {
-            const int j = ell[(size_t)e * m + i] - b;
+            const int j = ell[(((size_t)(e >> 2) * m + i) << 2) + (e & 3)] - b;
             if (j < 0 || j >= m) return fail(MPH_ERR_STATE, "neighbour index outside its case");
             row.push_back(idx[j]);
         }
@@ -765,6 +867,8 @@ extern "C" int mph_get_stats(const mph_ctx* c, mph_stats* out)
     for (int v : cnt) out->list_entries += v;
     out->list_width = c->width;
     out->max_list_len = mx;
+    out->sweep_a_min_waves = kSweepMinWaves[c->variant_a];
+    out->sweep_b_min_waves = kSweepMinWaves[c->variant_b];
     return MPH_OK;
 }
 

@@ -8,7 +8,8 @@
 //   aux   double4 {Gx, Gy, Gz, PA}  gravity centre and attractive pressure
 //   force double4 {Fx, Fy, Fz, 0}
 //   snap  double4                   positions at the last list build
-//   nbr   int [width][stride]       ELL neighbour lists, entry k of particle i at k*stride+i
+//   nbr   int [width/4][stride][4]  ELL neighbour lists in groups of four: entry k of particle i
+//                                   at ((k/4)*stride + i)*4 + k%4, one 16-byte load per group
 //
 // Every particle kernel is launched over a block map: block b handles particles
 // [map[b].y, min(map[b].y + blockDim, case.end)) of case map[b].x, so the case constants
@@ -347,6 +348,11 @@ k_scatter_back(const CaseConst* __restrict__ cases, const int2* __restrict__ bma
 // id, as the reference fills its lists.  Self is not stored; count_out carries the
 // reference's NeighborCount (list length including self).
 
+__device__ __forceinline__ size_t ell_slot(int k, int stride, int i)
+{
+    return ((size_t)(k >> 2) * stride + i) * 4 + (k & 3);
+}
+
 __device__ __forceinline__ int axis_window(int ic, int c, int range, int* out)
 {
     // the periodic window {ic-1, ic, ic+1} (or {ic} for range 0) in ascending order,
@@ -403,7 +409,7 @@ k_build_lists(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap
                             const double q2 = dx * dx + dy * dy + dz * dz;
                             if (q2 <= C.cut2) {
                                 if (j == i) { seen_self = 1; continue; }
-                                if (count < width) nbr[(size_t)count * stride + i] = j;
+                                if (count < width) nbr[ell_slot(count, stride, i)] = j;
                                 ++count;
                             }
                         }
@@ -418,29 +424,48 @@ k_build_lists(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap
 }
 
 // -------------------------------------------------------------------------------------
+// Partner loop shared by the sweeps.  List entries arrive four at a time (one 16-byte
+// load per group); the data of the next partner is loaded while the current one is
+// evaluated, alternating two register sets, so a wavefront always has loads in flight.
+// Entries past a particle's count are never dereferenced.
+
+template <class Load, class Eval>
+__device__ __forceinline__ void for_partners(const int* __restrict__ nbr, int stride, int i, int cnt,
+                                             Load load, Eval eval)
+{
+    const int4* __restrict__ e4 = reinterpret_cast<const int4*>(nbr);
+    const int ng = (cnt + 3) >> 2;
+    int4 cur = ng > 0 ? e4[i] : make_int4(i, i, i, i);
+    auto A = load(cnt > 0 ? cur.x : i);
+    for (int g = 0; g < ng; ++g) {
+        const int k = 4 * g;
+        const int4 nxt = g + 1 < ng ? e4[(size_t)(g + 1) * stride + i] : make_int4(i, i, i, i);
+        auto B = load(k + 1 < cnt ? cur.y : i);
+        eval(A);                                   // k < cnt holds inside the loop
+        A = load(k + 2 < cnt ? cur.z : i);
+        if (k + 1 < cnt) eval(B);
+        B = load(k + 3 < cnt ? cur.w : i);
+        if (k + 2 < cnt) eval(A);
+        A = load(k + 4 < cnt ? nxt.x : i);
+        if (k + 3 < cnt) eval(B);
+        cur = nxt;
+    }
+}
+
+// -------------------------------------------------------------------------------------
 // sweep A: number densities, gravity centre, volumetric strain, divergence
 // (main.cpp:1665-1727, 1831-1896) with the pointwise coefficients and pressures of
 // main.cpp:1631-1662, 1898-1906, 1729-1740 fused into the epilogue.
 //
 // Per pair: exact periodic difference (the division-free forms above), q^2, one
 // reciprocal square root, then the kernel polynomials with their constant prefactors
-// pulled out of the sums.  The loop is software pipelined: the list index two entries
-// ahead and the particle data one entry ahead are in flight while a pair is evaluated.
+// pulled out of the sums.  The kernels are evaluated branch-free with the factor
+// max(1 - r/h, 0), which is zero outside the support exactly as the reference's cut-offs.
 
 struct PairA {
     double4 p;
     double vx, vy, vz;
 };
-
-__device__ __forceinline__ PairA load_a(const double4* __restrict__ pos, const double4* vel, int j)
-{
-    PairA q;
-    q.p = pos[j];
-    q.vx = vel[j].x;          // P (vel.w) of partners is being written by this kernel
-    q.vy = vel[j].y;
-    q.vz = vel[j].z;
-    return q;
-}
 
 template <int SURF, bool WRAP>
 __device__ __forceinline__ void sweep_a_loop(const CaseConst& C, const double* tab, int ti,
@@ -450,48 +475,42 @@ __device__ __forceinline__ void sweep_a_loop(const CaseConst& C, const double* t
                                              double& na, double& gx, double& gy, double& gz,
                                              double& wsum, double& dsum, int& npair)
 {
-    int jn = cnt > 1 ? nbr[stride + i] : i;
-    PairA nx = load_a(pos, vel, cnt > 0 ? nbr[i] : i);
-    for (int k = 0; k < cnt; ++k) {
-        const PairA q = nx;
-        if (k + 1 < cnt) nx = load_a(pos, vel, jn);
-        if (k + 2 < cnt) jn = nbr[(size_t)(k + 2) * stride + i];
+    auto load = [&](int j) {
+        PairA q;
+        q.p = pos[j];
+        q.vx = vel[j].x;          // P (vel.w) of partners is being written by this kernel
+        q.vy = vel[j].y;
+        q.vz = vel[j].z;
+        return q;
+    };
+    auto eval = [&](const PairA& q) {
         const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(q.p.x, pi.x, C.hwid[0]);
         const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(q.p.y, pi.y, C.hwid[1]);
         const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(q.p.z, pi.z, C.hwid[2]);
         const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
         npair += q2 <= C.hmax2;
-        const bool in_a = C.ha2 - q2 >= 0.0;
-        const bool in_p = C.hp2 - q2 >= 0.0;
-        if (in_a || in_p) {
-            const double ir = rsqrt_nr(q2);
-            const double r = q2 * ir;
-            if (in_a) {
-                const double ratio = C.ratio_one ? 1.0 : tab[16 + ti * 4 + (tag_of(q.p.w) & 3)];
-                const double s = r * C.inv_ha;
-                const double om = 1.0 - s;
-                const double om2 = om * om;
-                na = fma(ratio * s, om2, na);
-                if (SURF) {
-                    const double wg = ratio * om2;
-                    gx = fma(dx, wg, gx);
-                    gy = fma(dy, wg, gy);
-                    gz = fma(dz, wg, gz);
-                }
-            }
-            if (in_p) {
-                const double om = fma(-r, C.inv_hp, 1.0);
-                wsum = fma(om, om, wsum);
-                const double ux = q.vx - vix, uy = q.vy - viy, uz = q.vz - viz;
-                const double ud = fma(uz, dz, fma(uy, dy, ux * dx));
-                dsum = fma(ud, om * ir, dsum);
-            }
+        const double ir = rsqrt_nr(q2);
+        const double r = q2 * ir;
+        const double sa = r * C.inv_ha;
+        const double oma = fmax(1.0 - sa, 0.0);
+        const double ratio = C.ratio_one ? 1.0 : tab[16 + ti * 4 + (tag_of(q.p.w) & 3)];
+        const double wa = ratio * (oma * oma);
+        na = fma(wa, sa, na);
+        if (SURF) {
+            gx = fma(dx, wa, gx);
+            gy = fma(dy, wa, gy);
+            gz = fma(dz, wa, gz);
         }
-    }
+        const double omp = fmax(fma(-r, C.inv_hp, 1.0), 0.0);
+        wsum = fma(omp, omp, wsum);
+        const double ud = fma(q.vz - viz, dz, fma(q.vy - viy, dy, (q.vx - vix) * dx));
+        dsum = fma(ud, omp * ir, dsum);
+    };
+    for_partners(nbr, stride, i, cnt, load, eval);
 }
 
-template <int SURF>
-__global__ void __launch_bounds__(256)
+template <int SURF, int MINW>
+__global__ void __launch_bounds__(256, MINW)
 k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
           const double4* __restrict__ pos, double4* vel, double4* __restrict__ aux,
           double* __restrict__ dens_a, double* __restrict__ vstrain, double* __restrict__ divp,
@@ -557,17 +576,6 @@ struct PairB {
     double4 p, v, a;
 };
 
-template <int SURF>
-__device__ __forceinline__ PairB load_b(const double4* __restrict__ pos, const double4* __restrict__ vel,
-                                        const double4* __restrict__ aux, int j)
-{
-    PairB q;
-    q.p = pos[j];
-    q.v = vel[j];
-    if (SURF) q.a = aux[j];
-    return q;
-}
-
 template <int SURF, bool WRAP>
 __device__ __forceinline__ void sweep_b_loop(const CaseConst& C, const double* tab, int ti,
                                              const double4& pi, const double4& vi, const double4& ai,
@@ -576,45 +584,43 @@ __device__ __forceinline__ void sweep_b_loop(const CaseConst& C, const double* t
                                              int stride, int i, int cnt, double3& fp, double3& fv,
                                              double3& fa, double3& fs)
 {
-    int jn = cnt > 1 ? nbr[stride + i] : i;
-    PairB nx = load_b<SURF>(pos, vel, aux, cnt > 0 ? nbr[i] : i);
-    for (int k = 0; k < cnt; ++k) {
-        const PairB q = nx;
-        if (k + 1 < cnt) nx = load_b<SURF>(pos, vel, aux, jn);
-        if (k + 2 < cnt) jn = nbr[(size_t)(k + 2) * stride + i];
+    const double mu_i = tab[ti * 4 + ti];
+    auto load = [&](int j) {
+        PairB q;
+        q.p = pos[j];
+        q.v = vel[j];
+        if (SURF) q.a = aux[j];
+        return q;
+    };
+    auto eval = [&](const PairB& q) {
         const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(q.p.x, pi.x, C.hwid[0]);
         const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(q.p.y, pi.y, C.hwid[1]);
         const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(q.p.z, pi.z, C.hwid[2]);
         const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        const bool in_p = C.hp2 - q2 > 0.0;
-        const bool in_v = C.hv2 - q2 > 0.0;
-        const bool in_a = SURF && (C.ha2 - q2 > 0.0);
-        if (!(in_p || in_v || in_a)) continue;
         const double ir = rsqrt_nr(q2);
         const double r = q2 * ir;
         const int tj = tag_of(q.p.w) & 3;
-        if (in_p) {
-            const double c = (vi.w + q.v.w) * fma(-r, C.inv_hp, 1.0) * ir;
-            fp.x = fma(c, dx, fp.x);
-            fp.y = fma(c, dy, fp.y);
-            fp.z = fma(c, dz, fp.z);
-        }
-        if (in_v) {
-            const double ud = fma(q.v.z - vi.z, dz, fma(q.v.y - vi.y, dy, (q.v.x - vi.x) * dx));
-            const double c = tab[ti * 4 + tj] * ud * fma(-r, C.inv_hv, 1.0) * (ir * ir * ir);
-            fv.x = fma(c, dx, fv.x);
-            fv.y = fma(c, dy, fv.y);
-            fv.z = fma(c, dz, fv.z);
-        }
-        if (SURF && in_a) {
+        // pressure
+        const double cp = (vi.w + q.v.w) * fmax(fma(-r, C.inv_hp, 1.0), 0.0) * ir;
+        fp.x = fma(cp, dx, fp.x);
+        fp.y = fma(cp, dy, fp.y);
+        fp.z = fma(cp, dz, fp.z);
+        // viscosity with the harmonic-mean coefficient of the type pair
+        const double mu = C.mu_uniform ? mu_i : tab[ti * 4 + tj];
+        const double ud = fma(q.v.z - vi.z, dz, fma(q.v.y - vi.y, dy, (q.v.x - vi.x) * dx));
+        const double cv = mu * ud * fmax(fma(-r, C.inv_hv, 1.0), 0.0) * (ir * ir * ir);
+        fv.x = fma(cv, dx, fv.x);
+        fv.y = fma(cv, dy, fv.y);
+        fv.z = fma(cv, dz, fv.z);
+        if (SURF) {
             const double rij = C.ratio_one ? 1.0 : tab[16 + ti * 4 + tj];
             const double rji = C.ratio_one ? 1.0 : tab[16 + tj * 4 + ti];
             const double s = r * C.inv_ha;
-            const double om = 1.0 - s;
-            const double c = fma(q.a.w, rji, ai.w * rij) * om * fma(-3.0, s, 1.0) * ir;
-            fa.x = fma(c, dx, fa.x);
-            fa.y = fma(c, dy, fa.y);
-            fa.z = fma(c, dz, fa.z);
+            const double om = fmax(1.0 - s, 0.0);
+            const double ca = fma(q.a.w, rji, ai.w * rij) * om * fma(-3.0, s, 1.0) * ir;
+            fa.x = fma(ca, dx, fa.x);
+            fa.y = fma(ca, dy, fa.y);
+            fa.z = fma(ca, dz, fa.z);
             // diffuse interface with Q = R_ji G_j - R_ij G_i; a_j uses the type of i
             // (main.cpp:1792), so a_i factors out of the sum
             const double qx = fma(rji, q.a.x, -rij * ai.x);
@@ -626,11 +632,12 @@ __device__ __forceinline__ void sweep_b_loop(const CaseConst& C, const double* t
             fs.y = fma(-om, fma(qd, dy, w * qy), fs.y);
             fs.z = fma(-om, fma(qd, dz, w * qz), fs.z);
         }
-    }
+    };
+    for_partners(nbr, stride, i, cnt, load, eval);
 }
 
-template <int SURF>
-__global__ void __launch_bounds__(256)
+template <int SURF, int MINW>
+__global__ void __launch_bounds__(256, MINW)
 k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
           const double4* __restrict__ pos, const double4* __restrict__ vel,
           const double4* __restrict__ aux, double4* __restrict__ vnext,
@@ -638,7 +645,7 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
           const int* __restrict__ ncnt, int stride, int* __restrict__ flags,
           int* __restrict__ any_flag, long long* __restrict__ rebuilds, int ncases)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && ncases > 0) {
         for (int c = 0; c < ncases; ++c) {
             if (flags[c]) rebuilds[c] += 1;
             flags[c] = 0;
@@ -722,7 +729,7 @@ k_virial(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
     double tp = 0.0, ta = 0.0, tv = 0.0, ts = 0.0;
     const int cnt = ncnt[i];
     for (int k = 0; k < cnt; ++k) {
-        const int j = nbr[(size_t)k * stride + i];
+        const int j = nbr[ell_slot(k, stride, i)];
         const double4 pj = pos[j];
         double dx, dy, dz;
         rel3(C, pi, pj, dx, dy, dz);
