@@ -134,7 +134,7 @@ def main():
         "config": {"workload": deck.name, "particles": deck.n, "fluid": int((deck.type < 2).sum()),
                    "precision": args.precision, "dt": deck.data.dt, "graph": bool(args.graph),
                    "list_width": st["list_width"],
-                   "sweep_min_waves": [st["sweep_a_min_waves"], st["sweep_b_min_waves"]]},
+                   "cluster_size": st["cluster_size"]},
         "pair_interactions_per_s": pairs_per_step * steps_per_s,
         "pairs_per_step": pairs_per_step,
         "simulated_seconds_per_s": deck.data.dt * steps_per_s,

@@ -93,11 +93,11 @@ typedef struct mph_fields {
 typedef struct mph_stats {
     long long steps;                  /* steps taken since create */
     long long rebuilds;               /* neighbour-list rebuilds (incl. the initial one) */
-    long long list_entries;           /* stored neighbour entries at the last rebuild (no self) */
-    int list_width;                   /* reserved entries per particle */
-    int max_list_len;                 /* longest list (without self) since create */
-    int sweep_a_min_waves;            /* occupancy variant of the density sweep picked by the autotuner */
-    int sweep_b_min_waves;            /* ... and of the force sweep */
+    long long list_entries;           /* entries of all cluster lists at the last rebuild */
+    int list_width;                   /* reserved entries per cluster list */
+    int max_list_len;                 /* longest cluster list since the lists were sized */
+    int cluster_size;                 /* particles per cluster tile, picked by the autotuner */
+    int clusters;                     /* clusters at the last rebuild */
 } mph_stats;
 
 /* Per-phase device time of mph_step_timed, summed over its steps. */
@@ -187,8 +187,9 @@ int mph_compute_virial(mph_ctx *ctx);
 
 /* Neighbour lists of the last rebuild in CSR form keyed by original particle id:
  * ptr[n+1], ind[total] with each row ascending and self included, i.e. the reference's
- * NeighborPtr/NeighborInd (main.cpp:1329-1596) mapped through ParticleIndex.  Call with
- * ind == NULL to get only ptr (ptr[n] = total). */
+ * NeighborPtr/NeighborInd (main.cpp:1329-1596) mapped through ParticleIndex.  The rows
+ * are recovered on the host from the cluster lists with the reference's exact distance
+ * test at the rebuild positions.  Call with ind == NULL to get only ptr (ptr[n] = total). */
 int mph_get_neighbors(mph_ctx *ctx, int case_id, long long *ptr, int *ind);
 
 int mph_get_stats(const mph_ctx *ctx, mph_stats *out);

@@ -63,8 +63,8 @@ class MphStats(C.Structure):
         ("list_entries", C.c_longlong),
         ("list_width", C.c_int),
         ("max_list_len", C.c_int),
-        ("sweep_a_min_waves", C.c_int),
-        ("sweep_b_min_waves", C.c_int),
+        ("cluster_size", C.c_int),
+        ("clusters", C.c_int),
     ]
 
 

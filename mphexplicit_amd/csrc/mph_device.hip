@@ -69,28 +69,35 @@ struct CaseHost {
 
 }  // namespace
 
+// cluster sizes the sweeps are instantiated for; the autotuner picks one per context
+constexpr int kClusterSizes[] = {2, 4, 8};
+constexpr int kNumClusterSizes = 3;
+
 struct mph_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    int ncases = 0, n = 0, stride = 0, width = 0;
+    int ncases = 0, n = 0, ncells = 0;
     int nkeys = 0, sort_passes = 0, sort_blocks = 0, any_surface = 0;
+    int K = 4;                 // cluster size (rows per wavefront)
+    int width = 0;             // reserved entries per cluster list
+    int nslots = 0;            // cluster slots over all cases (capacity)
     long long steps = 0;
     std::vector<CaseHost> cases;
-    std::vector<int2> bmap_host;
+    std::vector<int2> bmap_host;   // particle blocks (XCD-interleaved)
+    std::vector<int2> cmap_host;   // cluster blocks of 4 slots (XCD-interleaved)
     DevBuf<CaseConst> d_cases;
-    DevBuf<int2> d_bmap;
+    DevBuf<int2> d_bmap, d_cmap;
     DevBuf<double4> pos, pos_alt, vel, vel_alt, vnext, aux, force, snap;
-    DevBuf<int> pidx, pidx_alt, ncnt, nout, nbr, start;
+    DevBuf<int> pidx, pidx_alt, nout, start;
     DevBuf<double> dens_a, vstrain, divp, wallc, vir;
-    DevBuf<unsigned> keys_a, keys_b, hist;
-    DevBuf<int> perm_a, perm_b, rank, cell_of_rank;
+    DevBuf<unsigned> keys_a, keys_b, hist, ccount;
+    DevBuf<int> perm_a, perm_b, rank, cell_of_rank, cell_case, slot_case;
+    DevBuf<int4> cl_info;
+    DevBuf<int> cl_count, cl_nbr, cl_len;
     DevBuf<int> flags, any_flag, max_len;
     DevBuf<long long> rebuilds;
     DevBuf<unsigned long long> pairs, pairs_scratch;
-    DevBuf<double4> tune_v, tune_f;      // sweep B outputs while the autotuner times variants
-    int variant_a = 1, variant_b = 1;    // indices into the sweep kernel variant tables
-    double4* vnext_out(int real) { return real ? vnext.p : tune_v.p; }
-    double4* force_out(int real) { return real ? force.p : tune_f.p; }
+    DevBuf<double4> tune_v, tune_f;      // sweep B outputs while the autotuner times tiles
     int* h_scalar = nullptr;   // pinned host word for small readbacks
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_src = nullptr;
@@ -106,7 +113,24 @@ struct mph_ctx {
     unsigned* sorted_keys() { return (sort_passes & 1) ? keys_b.p : keys_a.p; }
     int* sorted_perm() { return (sort_passes & 1) ? perm_b.p : perm_a.p; }
     int nblocks() const { return (int)bmap_host.size(); }
+    int ncblocks() const { return (int)cmap_host.size(); }
+    double4* vnext_out(int real) { return real ? vnext.p : tune_v.p; }
+    double4* force_out(int real) { return real ? force.p : tune_f.p; }
 };
+
+// XCD-aware order: physical workgroup w runs on XCD w % kXcds; give each XCD one
+// contiguous run of the logical (space-ordered) blocks, padding with case -1
+static std::vector<int2> xcd_order(const std::vector<int2>& logical)
+{
+    const int nb = (int)logical.size();
+    const int per = (nb + kXcds - 1) / kXcds;
+    std::vector<int2> out((size_t)per * kXcds, make_int2(-1, 0));
+    for (int w = 0; w < per * kXcds; ++w) {
+        const int L = (w % kXcds) * per + w / kXcds;
+        if (L < nb) out[w] = logical[L];
+    }
+    return out;
+}
 
 // ---------------------------------------------------------------------------------------
 // host-side constants, main.cpp:836-1084
@@ -293,6 +317,39 @@ static int derive_case(const mph_params& p, int begin, int end, int nfluid, int 
 }
 
 // ---------------------------------------------------------------------------------------
+// launches
+
+#define LAUNCH_K(kern, K, ...)                                                        \
+    do {                                                                              \
+        switch (K) {                                                                  \
+        case 2: hipLaunchKernelGGL((kern<2>), __VA_ARGS__); break;                    \
+        case 4: hipLaunchKernelGGL((kern<4>), __VA_ARGS__); break;                    \
+        default: hipLaunchKernelGGL((kern<8>), __VA_ARGS__); break;                   \
+        }                                                                             \
+    } while (0)
+
+#define LAUNCH_SK(kern, SURF, K, ...)                                                 \
+    do {                                                                              \
+        if (SURF) {                                                                   \
+            switch (K) {                                                              \
+            case 2: hipLaunchKernelGGL((kern<1, 2>), __VA_ARGS__); break;             \
+            case 4: hipLaunchKernelGGL((kern<1, 4>), __VA_ARGS__); break;             \
+            default: hipLaunchKernelGGL((kern<1, 8>), __VA_ARGS__); break;            \
+            }                                                                         \
+        } else {                                                                      \
+            switch (K) {                                                              \
+            case 2: hipLaunchKernelGGL((kern<0, 2>), __VA_ARGS__); break;             \
+            case 4: hipLaunchKernelGGL((kern<0, 4>), __VA_ARGS__); break;             \
+            default: hipLaunchKernelGGL((kern<0, 8>), __VA_ARGS__); break;            \
+            }                                                                         \
+        }                                                                             \
+    } while (0)
+
+static int launch_ok(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MPH_OK : fail(MPH_ERR_DEVICE, "%s launch failed: %s", what, hipGetErrorString(e));
+}
 
 static int launch_rebuild(mph_ctx* c, int count_only)
 {
@@ -322,54 +379,46 @@ static int launch_rebuild(mph_ctx* c, int count_only)
     hipLaunchKernelGGL(k_scatter_back, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p,
                        c->flags.p, c->any_flag.p, c->pos_alt.p, c->vel_alt.p, c->pidx_alt.p, c->pos.p,
                        c->vel.p, c->pidx.p, c->snap.p);
-    hipLaunchKernelGGL(k_build_lists, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p,
-                       c->flags.p, c->any_flag.p, c->pos.p, kf, c->start.p, c->rank.p,
-                       c->cell_of_rank.p, c->nbr.p, c->stride,
-                       count_only ? 0 : c->width, c->ncnt.p, c->nout.p, c->max_len.p);
-    return hipGetLastError() == hipSuccess ? MPH_OK : fail(MPH_ERR_DEVICE, "rebuild launch failed");
+    const int gcells = (c->ncells + 255) / 256;
+    hipLaunchKernelGGL(k_cluster_count, dim3(gcells), dim3(256), 0, s, c->d_cases.p, c->cell_case.p,
+                       c->ncells, c->start.p, c->K, c->ccount.p, c->any_flag.p);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, c->ccount.p, c->ncells, c->any_flag.p);
+    hipLaunchKernelGGL(k_cluster_fill, dim3(gcells), dim3(256), 0, s, c->d_cases.p, c->cell_case.p,
+                       c->ncells, c->start.p, c->cell_of_rank.p, c->K, c->ccount.p, c->cl_info.p,
+                       c->cl_count.p, c->any_flag.p);
+    const long threads = (long)c->nslots * c->K;
+    hipLaunchKernelGGL(k_cluster_lists, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, c->d_cases.p,
+                       c->slot_case.p, c->nslots, c->flags.p, c->any_flag.p, c->pos.p, c->start.p, c->rank.p,
+                       c->cl_info.p, c->cl_count.p, c->K, count_only ? 0 : c->width, c->cl_nbr.p,
+                       c->cl_len.p, c->nout.p, c->max_len.p);
+    return launch_ok("rebuild");
 }
 
-// sweep kernel variants: minimum waves per SIMD the register allocation must allow
-// (the autotuner picks one per context by timing them on the case itself)
-constexpr int kSweepVariants = 4;
-static const int kSweepMinWaves[kSweepVariants] = {4, 5, 6, 8};
-using SweepAFn = decltype(&k_sweep_a<0, 4>);
-using SweepBFn = decltype(&k_sweep_b<0, 4>);
-static const SweepAFn kSweepA[2][kSweepVariants] = {
-    {k_sweep_a<0, 4>, k_sweep_a<0, 5>, k_sweep_a<0, 6>, k_sweep_a<0, 8>},
-    {k_sweep_a<1, 4>, k_sweep_a<1, 5>, k_sweep_a<1, 6>, k_sweep_a<1, 8>}};
-static const SweepBFn kSweepB[2][kSweepVariants] = {
-    {k_sweep_b<0, 4>, k_sweep_b<0, 5>, k_sweep_b<0, 6>, k_sweep_b<0, 8>},
-    {k_sweep_b<1, 4>, k_sweep_b<1, 5>, k_sweep_b<1, 6>, k_sweep_b<1, 8>}};
-
-// advance_walls: this launch is a real step (advances wall centres, counts pairs); the
-// autotuner launches with 0 so that repeated launches leave the state unchanged
-static int launch_sweep_a(mph_ctx* c, int advance_walls, int variant = -1)
+// real: a step (advances wall centres, counts pairs); the autotuner launches with 0 so
+// that repeated launches leave the state unchanged
+static int launch_sweep_a(mph_ctx* c, int real)
 {
-    const int v = variant >= 0 ? variant : c->variant_a;
-    const int ncw = advance_walls ? c->ncases : 0;
-    hipLaunchKernelGGL(kSweepA[c->any_surface ? 1 : 0][v], dim3(c->nblocks()), dim3(kBlock), 0, c->stream,
-                       c->d_cases.p, c->d_bmap.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p,
-                       c->divp.p, c->nbr.p, c->ncnt.p, c->stride, c->wallc.p, ncw,
-                       advance_walls ? c->pairs.p : c->pairs_scratch.p);
-    return hipGetLastError() == hipSuccess ? MPH_OK : fail(MPH_ERR_DEVICE, "sweep A launch failed");
+    LAUNCH_SK(k_sweep_a, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p,
+              c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p,
+              c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p, c->wallc.p,
+              real ? c->ncases : 0, real ? c->pairs.p : c->pairs_scratch.p);
+    return launch_ok("sweep A");
+}
+
+static int launch_sweep_b(mph_ctx* c, int real)
+{
+    LAUNCH_SK(k_sweep_b, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p,
+              c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p,
+              c->pos.p, c->vel.p, c->aux.p, c->vnext_out(real), c->force_out(real), c->flags.p,
+              c->any_flag.p, c->rebuilds.p, real ? c->ncases : 0);
+    return launch_ok("sweep B");
 }
 
 static int launch_move(mph_ctx* c)
 {
     hipLaunchKernelGGL(k_move, dim3(c->nblocks()), dim3(kBlock), 0, c->stream, c->d_cases.p, c->d_bmap.p,
                        c->pos.p, c->vel.p, c->vnext.p, c->snap.p, c->wallc.p, c->flags.p, c->any_flag.p);
-    return hipGetLastError() == hipSuccess ? MPH_OK : fail(MPH_ERR_DEVICE, "move launch failed");
-}
-
-static int launch_sweep_b(mph_ctx* c, int bookkeeping = 1, int variant = -1)
-{
-    const int v = variant >= 0 ? variant : c->variant_b;
-    hipLaunchKernelGGL(kSweepB[c->any_surface ? 1 : 0][v], dim3(c->nblocks()), dim3(kBlock), 0, c->stream,
-                       c->d_cases.p, c->d_bmap.p, c->pos.p, c->vel.p, c->aux.p, c->vnext_out(bookkeeping),
-                       c->force_out(bookkeeping), c->nbr.p, c->ncnt.p, c->stride, c->flags.p, c->any_flag.p,
-                       c->rebuilds.p, bookkeeping ? c->ncases : 0);
-    return hipGetLastError() == hipSuccess ? MPH_OK : fail(MPH_ERR_DEVICE, "sweep B launch failed");
+    return launch_ok("move");
 }
 
 static int launch_step(mph_ctx* c)
@@ -377,7 +426,7 @@ static int launch_step(mph_ctx* c)
     int rc = launch_move(c);
     if (!rc) rc = launch_rebuild(c, 0);
     if (!rc) rc = launch_sweep_a(c, 1);
-    if (!rc) rc = launch_sweep_b(c);
+    if (!rc) rc = launch_sweep_b(c, 1);
     return rc;
 }
 
@@ -395,16 +444,64 @@ static int check_capacity(mph_ctx* c)
     int rc = read_int(c, c->max_len.p, &mx);
     if (rc) return rc;
     if (mx > c->width)
-        return fail(MPH_ERR_CAPACITY, "a neighbour list reached %d entries, above the reserved width %d; "
+        return fail(MPH_ERR_CAPACITY, "a cluster neighbour list reached %d entries, above the reserved width %d; "
                     "results of this call are invalid", mx, c->width);
     return MPH_OK;
 }
 
+// Cluster slots, block map and lists for cluster size K: each case reserves
+// ceil(n/K) + cells slots (every cell adds at most one partial cluster).  All cases are
+// flagged for a rebuild, the union lists are counted to size their width, then built.
+static int prepare_clusters(mph_ctx* c, int K)
+{
+    c->K = K;
+    std::vector<int> hslot_case;
+    std::vector<int2> logical;
+    int base = 0;
+    for (int k = 0; k < c->ncases; ++k) {
+        CaseConst& kc = c->cases[k].k;
+        const int nk = kc.end - kc.begin;
+        kc.cl_base = base;
+        kc.cl_cap = (nk + K - 1) / K + kc.tcc;
+        for (int q = 0; q < kc.cl_cap; ++q) hslot_case.push_back(k);
+        for (int q = 0; q < kc.cl_cap; q += 4) logical.push_back(make_int2(k, q));
+        base += kc.cl_cap;
+    }
+    c->nslots = base;
+    c->cmap_host = xcd_order(logical);
+    std::vector<CaseConst> hk(c->ncases);
+    for (int k = 0; k < c->ncases; ++k) hk[k] = c->cases[k].k;
+    int rc = 0;
+    if ((rc = c->slot_case.alloc(c->nslots)) || (rc = c->cl_info.alloc(c->nslots)) ||
+        (rc = c->cl_len.alloc(c->nslots)) || (rc = c->d_cmap.alloc(c->cmap_host.size())))
+        return rc;
+    hipStream_t s = c->stream;
+    HIP_TRY(hipMemcpyAsync(c->d_cases.p, hk.data(), sizeof(CaseConst) * c->ncases, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->slot_case.p, hslot_case.data(), sizeof(int) * c->nslots, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->d_cmap.p, c->cmap_host.data(), sizeof(int2) * c->cmap_host.size(),
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(c->cl_len.p, 0, sizeof(int) * c->nslots, s));
+    std::vector<int> ones(c->ncases, 1);
+    HIP_TRY(hipMemcpyAsync(c->flags.p, ones.data(), sizeof(int) * c->ncases, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->any_flag.p, ones.data(), sizeof(int), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(c->max_len.p, 0, sizeof(int), s));
+    if ((rc = c->cl_nbr.alloc(1))) return rc;
+    c->width = 0;
+    if ((rc = launch_rebuild(c, 1))) return rc;
+    int mx = 0;
+    if ((rc = read_int(c, c->max_len.p, &mx))) return rc;
+    c->width = ((int)std::ceil(mx * 1.25) + 16 + 15) / 16 * 16;
+    if ((rc = c->cl_nbr.alloc((size_t)c->width * c->nslots))) return rc;
+    HIP_TRY(hipMemsetAsync(c->max_len.p, 0, sizeof(int), s));
+    return launch_rebuild(c, 0);      // the sort is the identity now
+}
+
 // ---------------------------------------------------------------------------------------
-// autotuner: time every sweep variant on the context's own lists and keep the fastest.
-// Results are cached in a text file (MPH_TUNE_CACHE, default ~/.cache/mphexplicit_amd/
-// sweep_variants.txt) keyed by device, dimension, surface tension and problem size;
-// MPH_TUNE=0 keeps the built-in defaults, MPH_TUNE=force ignores the cache.
+// autotuner: time the sweeps for every cluster size on the context's own particles and
+// keep the fastest.  Results are cached in a text file (MPH_TUNE_CACHE, default
+// ~/.cache/mphexplicit_amd/cluster_size.txt) keyed by device, dimension, surface tension
+// and problem size; MPH_TUNE=0 keeps K = 4, MPH_TUNE=<2|4|8> forces a size and
+// MPH_TUNE=force ignores the cache.
 
 static std::string tune_key(mph_ctx* c)
 {
@@ -416,8 +513,11 @@ static std::string tune_key(mph_ctx* c)
     int nb = 0;
     while ((1ll << (nb + 1)) <= (long long)c->n) ++nb;
     char buf[256];
-    std::snprintf(buf, sizeof(buf), "%s|dim%d|surf%d|n2^%d|w%d", arch.c_str(), dim, c->any_surface, nb, c->width / 16);
-    return buf;
+    std::snprintf(buf, sizeof(buf), "%s|dim%d|surf%d|cases%d|n2^%d", arch.c_str(), dim, c->any_surface,
+                  c->ncases, nb);
+    std::string k = buf;
+    for (auto& ch : k) if (ch == ' ') ch = '_';
+    return k;
 }
 
 static std::string tune_path()
@@ -425,22 +525,49 @@ static std::string tune_path()
     const char* p = std::getenv("MPH_TUNE_CACHE");
     if (p && *p) return p;
     const char* h = std::getenv("HOME");
-    return std::string(h ? h : "/tmp") + "/.cache/mphexplicit_amd/sweep_variants.txt";
+    return std::string(h ? h : "/tmp") + "/.cache/mphexplicit_amd/cluster_size.txt";
 }
 
-static int autotune(mph_ctx* c)
+static int time_sweeps(mph_ctx* c, float* ms_out)
+{
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = MPH_OK;
+    float total = 0.0f;
+    for (int rep = 0; rep < 4 && rc == MPH_OK; ++rep) {
+        (void)hipEventRecord(e0, c->stream);
+        rc = launch_sweep_a(c, 0);
+        if (!rc) rc = launch_sweep_b(c, 0);
+        (void)hipEventRecord(e1, c->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0) total += ms;           // the first launch warms caches
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_out = total;
+    return rc;
+}
+
+static int choose_cluster_size(mph_ctx* c, int* K)
 {
     const char* mode = std::getenv("MPH_TUNE");
+    *K = 4;
     if (mode && !std::strcmp(mode, "0")) return MPH_OK;
+    if (mode && (!std::strcmp(mode, "2") || !std::strcmp(mode, "4") || !std::strcmp(mode, "8"))) {
+        *K = std::atoi(mode);
+        return MPH_OK;
+    }
     const std::string key = tune_key(c), path = tune_path();
     if (!(mode && !std::strcmp(mode, "force"))) {
         if (FILE* f = std::fopen(path.c_str(), "r")) {
             char k[256];
-            int a, b;
-            while (std::fscanf(f, "%255s %d %d", k, &a, &b) == 3)
-                if (key == k && a >= 0 && a < kSweepVariants && b >= 0 && b < kSweepVariants) {
-                    c->variant_a = a;
-                    c->variant_b = b;
+            int v;
+            while (std::fscanf(f, "%255s %d", k, &v) == 2)
+                if (key == k && (v == 2 || v == 4 || v == 8)) {
+                    *K = v;
                     std::fclose(f);
                     return MPH_OK;
                 }
@@ -449,28 +576,14 @@ static int autotune(mph_ctx* c)
     }
     int rc = 0;
     if ((rc = c->tune_v.alloc(c->n)) || (rc = c->tune_f.alloc(c->n))) return rc;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    float best_a = 1e30f, best_b = 1e30f;
-    for (int v = 0; v < kSweepVariants && rc == MPH_OK; ++v) {
-        for (int kind = 0; kind < 2 && rc == MPH_OK; ++kind) {
-            float total = 0.0f;
-            for (int rep = 0; rep < 4 && rc == MPH_OK; ++rep) {
-                (void)hipEventRecord(e0, c->stream);
-                rc = kind == 0 ? launch_sweep_a(c, 0, v) : launch_sweep_b(c, 0, v);
-                (void)hipEventRecord(e1, c->stream);
-                (void)hipEventSynchronize(e1);
-                float ms = 0.0f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (rep > 0) total += ms;           // the first launch warms caches
-            }
-            if (kind == 0 && total < best_a) { best_a = total; c->variant_a = v; }
-            if (kind == 1 && total < best_b) { best_b = total; c->variant_b = v; }
-        }
+    float best = 1e30f;
+    for (int q = 0; q < kNumClusterSizes && rc == MPH_OK; ++q) {
+        rc = prepare_clusters(c, kClusterSizes[q]);
+        if (!rc) rc = launch_sweep_a(c, 0);     // densities and pressures for sweep B's inputs
+        float ms = 0.0f;
+        if (!rc) rc = time_sweeps(c, &ms);
+        if (!rc && ms < best) { best = ms; *K = kClusterSizes[q]; }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     c->tune_v.release();
     c->tune_f.release();
     if (rc) return rc;
@@ -478,7 +591,7 @@ static int autotune(mph_ctx* c)
     std::string mk = "mkdir -p '" + dir + "' 2>/dev/null";
     (void)!std::system(mk.c_str());
     if (FILE* f = std::fopen(path.c_str(), "a")) {
-        std::fprintf(f, "%s %d %d\n", key.c_str(), c->variant_a, c->variant_b);
+        std::fprintf(f, "%s %d\n", key.c_str(), *K);
         std::fclose(f);
     }
     return MPH_OK;
@@ -495,7 +608,6 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     HIP_TRY(hipHostMalloc((void**)&c->h_scalar, sizeof(long long) * 4, hipHostMallocDefault));
     c->ncases = ncases;
     c->cases.resize(ncases);
-    // particles of each case, fluid first (the reference's order after its first sort)
     long long total = 0;
     for (int k = 0; k < ncases; ++k) {
         if (counts[k] <= 0) return fail(MPH_ERR_ARG, "case %d has no particles", k);
@@ -503,12 +615,11 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     }
     if (total > (1ll << 30)) return fail(MPH_ERR_ARG, "too many particles");
     c->n = (int)total;
-    c->stride = (c->n + 63) / 64 * 64;
     std::vector<double4> hpos(c->n), hvel(c->n);
     std::vector<int> hidx(c->n);
     std::vector<double> hwall((size_t)ncases * kTypes * 3, 0.0);
     int begin = 0, key_base = 0;
-    std::vector<int> hrank, hcell;
+    std::vector<int> hrank, hcell, hcell_case;
     std::vector<int2> logical;
     c->any_surface = 0;
     for (int k = 0; k < ncases; ++k) {
@@ -523,7 +634,8 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         ch.prm = prm[k];
         int rc = derive_case(prm[k], begin, begin + nk, nfluid, key_base, ch.k);
         if (rc) return fail(rc, "case %d: %s", k, mph::last_error().c_str());
-        if ((long long)key_base + 2ll * ch.k.tcc > (1ll << 31) - 2) return fail(MPH_ERR_ARG, "cell keys overflow");
+        if (((long long)key_base + 2ll * ch.k.tcc + 1) << kSubBits >= (1ll << 32))
+            return fail(MPH_ERR_ARG, "too many cells for 32-bit sort keys");
         key_base += 2 * ch.k.tcc;
         ch.time = prm[k].time;
         c->any_surface |= ch.k.surface;
@@ -531,6 +643,11 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
             for (int d = 0; d < 3; ++d) hwall[((size_t)k * kTypes + t) * 3 + d] = prm[k].wall_center[t][d];
         for (int i = 0; i < nk; ++i) {
             const int g = begin + i;
+            for (int d = 0; d < 3; ++d) {
+                const double xv = x[3 * (size_t)g + d];
+                if (!(xv >= ch.k.dmin[d] - 0.25 * ch.k.dwid[d] && xv <= ch.k.dmax[d] + 0.25 * ch.k.dwid[d]))
+                    return fail(MPH_ERR_ARG, "case %d particle %d lies outside the periodic domain", k, i);
+            }
             const long long tag = (long long)(type[g] | (k << 2));
             double w;
             std::memcpy(&w, &tag, sizeof(w));
@@ -538,57 +655,41 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
             hvel[g] = make_double4(v[3 * g], v[3 * g + 1], v[3 * g + 2], 0.0);
             hidx[g] = i;
         }
-        for (int i = 0; i < nk; ++i)
-            for (int d = 0; d < 3; ++d) {
-                const double xv = x[3 * (size_t)(begin + i) + d];
-                if (!(xv >= ch.k.dmin[d] - 0.25 * ch.k.dwid[d] && xv <= ch.k.dmax[d] + 0.25 * ch.k.dwid[d]))
-                    return fail(MPH_ERR_ARG, "case %d particle %d lies outside the periodic domain", k, i);
-            }
         std::vector<int> rk, cr;
         morton_ranks(ch.k, rk, cr);
         ch.k.rank_base = (int)hrank.size();
         hrank.insert(hrank.end(), rk.begin(), rk.end());
         hcell.insert(hcell.end(), cr.begin(), cr.end());
+        hcell_case.insert(hcell_case.end(), (size_t)ch.k.tcc, k);
         for (int b = 0; b < nk; b += kBlock) logical.push_back(make_int2(k, begin + b));
         begin += nk;
     }
-    // XCD-aware order: physical workgroup w runs on XCD w % kXcds; give each XCD one
-    // contiguous run of the logical (particle-ordered) blocks
-    {
-        const int nb = (int)logical.size();
-        const int per = (nb + kXcds - 1) / kXcds;
-        c->bmap_host.assign((size_t)per * kXcds, make_int2(-1, 0));
-        for (int w = 0; w < per * kXcds; ++w) {
-            const int L = (w % kXcds) * per + w / kXcds;
-            if (L < nb) c->bmap_host[w] = logical[L];
-        }
-    }
+    c->bmap_host = xcd_order(logical);
+    c->ncells = (int)hrank.size();
     c->nkeys = key_base;
     int bits = 1;
-    while ((1ll << bits) < (long long)c->nkeys) ++bits;
+    while ((1ll << bits) < ((long long)c->nkeys << kSubBits)) ++bits;
     c->sort_passes = (bits + kRadixBits - 1) / kRadixBits;
     c->sort_blocks = (c->n + kSortTile - 1) / kSortTile;
 
-    std::vector<CaseConst> hk(ncases);
-    for (int k = 0; k < ncases; ++k) hk[k] = c->cases[k].k;
     int rc = 0;
     const size_t n = c->n;
     if ((rc = c->d_cases.alloc(ncases)) || (rc = c->d_bmap.alloc(c->bmap_host.size())) ||
         (rc = c->pos.alloc(n)) || (rc = c->pos_alt.alloc(n)) || (rc = c->vel.alloc(n)) ||
         (rc = c->vel_alt.alloc(n)) || (rc = c->vnext.alloc(n)) || (rc = c->aux.alloc(n)) ||
         (rc = c->force.alloc(n)) || (rc = c->snap.alloc(n)) || (rc = c->pidx.alloc(n)) ||
-        (rc = c->pidx_alt.alloc(n)) || (rc = c->ncnt.alloc(n)) || (rc = c->nout.alloc(n)) ||
+        (rc = c->pidx_alt.alloc(n)) || (rc = c->nout.alloc(n)) ||
         (rc = c->start.alloc((size_t)c->nkeys + 1)) || (rc = c->dens_a.alloc(n)) ||
-        (rc = c->vstrain.alloc(n)) || (rc = c->divp.alloc(n)) || (rc = c->vir.alloc(n)) || (rc = c->wallc.alloc(hwall.size())) ||
-        (rc = c->keys_a.alloc(n)) || (rc = c->keys_b.alloc(n)) ||
+        (rc = c->vstrain.alloc(n)) || (rc = c->divp.alloc(n)) || (rc = c->vir.alloc(n)) ||
+        (rc = c->wallc.alloc(hwall.size())) || (rc = c->keys_a.alloc(n)) || (rc = c->keys_b.alloc(n)) ||
         (rc = c->hist.alloc((size_t)kRadix * c->sort_blocks)) || (rc = c->perm_a.alloc(n)) ||
-        (rc = c->perm_b.alloc(n)) || (rc = c->flags.alloc(ncases)) ||
-        (rc = c->rank.alloc(hrank.size())) || (rc = c->cell_of_rank.alloc(hcell.size())) || (rc = c->any_flag.alloc(1)) ||
-        (rc = c->max_len.alloc(1)) || (rc = c->rebuilds.alloc(ncases)) || (rc = c->pairs.alloc(1)) ||
-        (rc = c->pairs_scratch.alloc(1)))
+        (rc = c->perm_b.alloc(n)) || (rc = c->flags.alloc(ncases)) || (rc = c->any_flag.alloc(1)) ||
+        (rc = c->rank.alloc(hrank.size())) || (rc = c->cell_of_rank.alloc(hcell.size())) ||
+        (rc = c->cell_case.alloc(c->ncells)) || (rc = c->ccount.alloc(c->ncells)) ||
+        (rc = c->cl_count.alloc(ncases)) || (rc = c->max_len.alloc(1)) ||
+        (rc = c->rebuilds.alloc(ncases)) || (rc = c->pairs.alloc(1)) || (rc = c->pairs_scratch.alloc(1)))
         return rc;
     hipStream_t s = c->stream;
-    HIP_TRY(hipMemcpyAsync(c->d_cases.p, hk.data(), sizeof(CaseConst) * ncases, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->d_bmap.p, c->bmap_host.data(), sizeof(int2) * c->bmap_host.size(), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->pos.p, hpos.data(), sizeof(double4) * n, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->vel.p, hvel.data(), sizeof(double4) * n, hipMemcpyHostToDevice, s));
@@ -596,28 +697,17 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     HIP_TRY(hipMemcpyAsync(c->wallc.p, hwall.data(), sizeof(double) * hwall.size(), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->rank.p, hrank.data(), sizeof(int) * hrank.size(), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->cell_of_rank.p, hcell.data(), sizeof(int) * hcell.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->cell_case.p, hcell_case.data(), sizeof(int) * hcell_case.size(), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(c->aux.p, 0, sizeof(double4) * n, s));
     HIP_TRY(hipMemsetAsync(c->force.p, 0, sizeof(double4) * n, s));
     HIP_TRY(hipMemsetAsync(c->vir.p, 0, sizeof(double) * n, s));
     HIP_TRY(hipMemsetAsync(c->rebuilds.p, 0, sizeof(long long) * ncases, s));
-    HIP_TRY(hipMemsetAsync(c->ncnt.p, 0, sizeof(int) * n, s));
-    HIP_TRY(hipMemsetAsync(c->max_len.p, 0, sizeof(int), s));
     HIP_TRY(hipMemsetAsync(c->pairs.p, 0, sizeof(unsigned long long), s));
-    // first bucketing and lists for every case (main.cpp:358-359): count pass to size the
-    // list width, then the real build
-    std::vector<int> ones(ncases, 1);
-    HIP_TRY(hipMemcpyAsync(c->flags.p, ones.data(), sizeof(int) * ncases, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->any_flag.p, ones.data(), sizeof(int), hipMemcpyHostToDevice, s));
-    // the list buffer is needed by the count pass only as a valid pointer
-    if ((rc = c->nbr.alloc(1))) return rc;
-    if ((rc = launch_rebuild(c, 1))) return rc;
-    int mx = 0;
-    if ((rc = read_int(c, c->max_len.p, &mx))) return rc;
-    c->width = ((int)std::ceil(mx * 1.25) + 8 + 3) / 4 * 4;
-    if ((rc = c->nbr.alloc((size_t)c->width * c->stride))) return rc;
-    HIP_TRY(hipMemsetAsync(c->max_len.p, 0, sizeof(int), s));
-    // the lists again with room for entries (the sort is the identity now)
-    if ((rc = launch_rebuild(c, 0))) return rc;
+    // first bucketing and lists (main.cpp:358-359), for the cluster size the autotuner
+    // picks (it builds and times every size, the last build is then redone for the winner)
+    int K = 4;
+    if ((rc = choose_cluster_size(c, &K))) return rc;
+    if ((rc = prepare_clusters(c, K))) return rc;
     if ((rc = launch_sweep_a(c, 0))) return rc;                          // main.cpp:362-364
     HIP_TRY(hipMemcpyAsync(c->vnext.p, c->vel.p, sizeof(double4) * n, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(c->flags.p, 0, sizeof(int) * ncases, s));
@@ -625,8 +715,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     std::vector<long long> one_ll(ncases, 1);
     HIP_TRY(hipMemcpyAsync(c->rebuilds.p, one_ll.data(), sizeof(long long) * ncases, hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
-    if ((rc = check_capacity(c))) return rc;
-    return autotune(c);
+    return check_capacity(c);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -710,7 +799,7 @@ extern "C" int mph_step_timed(mph_ctx* c, int nsteps, mph_timing* out)
         (void)hipEventRecord(e[2], c->stream);
         if (!rc) rc = launch_sweep_a(c, 1);
         (void)hipEventRecord(e[3], c->stream);
-        if (!rc) rc = launch_sweep_b(c);
+        if (!rc) rc = launch_sweep_b(c, 1);
         (void)hipEventRecord(e[4], c->stream);
         for (auto& ch : c->cases) ch.time += ch.prm.dt;
         ++c->steps;
@@ -807,10 +896,17 @@ extern "C" int mph_get_fields(mph_ctx* c, int case_id, int order, mph_fields* ou
 extern "C" int mph_compute_virial(mph_ctx* c)
 {
     if (!c) return fail(MPH_ERR_ARG, "null context");
-    hipLaunchKernelGGL(k_virial, dim3(c->nblocks()), dim3(kBlock), 0, c->stream, c->d_cases.p, c->d_bmap.p,
-                       c->pos.p, c->vel.p, c->vnext.p, c->aux.p, c->nbr.p, c->ncnt.p, c->stride, c->vir.p);
-    HIP_TRY(hipGetLastError());
-    return MPH_OK;
+    LAUNCH_K(k_virial, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p, c->d_cmap.p,
+             c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p, c->pos.p, c->vel.p,
+             c->vnext.p, c->aux.p, c->vir.p);
+    return launch_ok("virial");
+}
+
+// the reference's periodic difference (main.cpp:70, 1398) on the host, for list export
+static double host_rel(double xj, double xi, double hw, double w)
+{
+    const double a = xj - xi + hw;
+    return (a - w * std::floor(a / w)) - hw;
 }
 
 extern "C" int mph_get_neighbors(mph_ctx* c, int case_id, long long* ptr, int* ind)
@@ -818,57 +914,78 @@ extern "C" int mph_get_neighbors(mph_ctx* c, int case_id, long long* ptr, int* i
     if (!c || !ptr || case_id < 0 || case_id >= c->ncases) return fail(MPH_ERR_ARG, "mph_get_neighbors: bad argument");
     const CaseConst& k = c->cases[case_id].k;
     const int b = k.begin, m = k.end - k.begin;
-    std::vector<int> idx, cnt, nc, ell((size_t)c->width * m);
+    std::vector<int> idx, nc, len;
+    std::vector<double4> sn;
+    std::vector<int4> info;
     int rc = 0;
-    if ((rc = fetch(c, c->pidx.p, b, m, idx)) || (rc = fetch(c, c->ncnt.p, b, m, cnt)) ||
-        (rc = fetch(c, c->nout.p, b, m, nc)))
+    if ((rc = fetch(c, c->pidx.p, b, m, idx)) || (rc = fetch(c, c->nout.p, b, m, nc)) ||
+        (rc = fetch(c, c->snap.p, b, m, sn)) || (rc = fetch(c, c->cl_info.p, k.cl_base, k.cl_cap, info)) ||
+        (rc = fetch(c, c->cl_len.p, k.cl_base, k.cl_cap, len)))
         return rc;
-    // groups of four entries: rows of the 2D copy are groups, columns the case's particles
-    HIP_TRY(hipMemcpy2DAsync(ell.data(), sizeof(int) * 4 * m, c->nbr.p + (size_t)4 * b, sizeof(int) * 4 * c->stride,
-                             sizeof(int) * 4 * m, c->width / 4, hipMemcpyDeviceToHost, c->stream));
+    int ncl = 0;
+    HIP_TRY(hipMemcpyAsync(c->h_scalar, c->cl_count.p + case_id, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    // rows by original id: reference lists include self (when the particle has a list)
+    ncl = *c->h_scalar;
+    std::vector<int> lists((size_t)std::max(ncl, 1) * c->width);
+    if (ncl > 0)
+        HIP_TRY(hipMemcpy(lists.data(), c->cl_nbr.p + (size_t)k.cl_base * c->width,
+                          sizeof(int) * (size_t)ncl * c->width, hipMemcpyDeviceToHost));
+    // rows by original id: the reference's lists include self (when the particle has one)
     std::vector<long long> rowlen(m, 0);
     for (int i = 0; i < m; ++i) rowlen[idx[i]] = nc[i];
     ptr[0] = 0;
     for (int o = 0; o < m; ++o) ptr[o + 1] = ptr[o] + rowlen[o];
     if (!ind) return MPH_OK;
-    for (int i = 0; i < m; ++i) {
-        const int o = idx[i];
-        long long w = ptr[o];
-        std::vector<int> row;
-        for (int e = 0; e < cnt[i]; ++e) {
-            const int j = ell[(((size_t)(e >> 2) * m + i) << 2) + (e & 3)] - b;
-            if (j < 0 || j >= m) return fail(MPH_ERR_STATE, "neighbour index outside its case");
-            row.push_back(idx[j]);
+    for (int g = 0; g < ncl; ++g) {
+        const int4 in = info[g];
+        for (int r = 0; r < in.y; ++r) {
+            const int il = in.x + r - b;
+            if (il < 0 || il >= m) return fail(MPH_ERR_STATE, "cluster member outside its case");
+            const int o = idx[il];
+            if (nc[il] == 0) continue;
+            std::vector<int> row;
+            row.push_back(o);
+            for (int e = 0; e < len[g]; ++e) {
+                const int jl = lists[(size_t)g * c->width + e] - b;
+                if (jl < 0 || jl >= m) return fail(MPH_ERR_STATE, "neighbour index outside its case");
+                if (jl == il) continue;
+                const double dx = host_rel(sn[jl].x, sn[il].x, k.hwid[0], k.dwid[0]);
+                const double dy = host_rel(sn[jl].y, sn[il].y, k.hwid[1], k.dwid[1]);
+                const double dz = host_rel(sn[jl].z, sn[il].z, k.hwid[2], k.dwid[2]);
+                if (dx * dx + dy * dy + dz * dz <= k.cut2) row.push_back(idx[jl]);
+            }
+            std::sort(row.begin(), row.end());
+            if ((long long)row.size() != rowlen[o])
+                return fail(MPH_ERR_STATE, "neighbour row length mismatch (%zu vs %lld)", row.size(), rowlen[o]);
+            std::copy(row.begin(), row.end(), ind + ptr[o]);
         }
-        if (nc[i] > cnt[i]) row.push_back(o);
-        std::sort(row.begin(), row.end());
-        if ((long long)row.size() != rowlen[o]) return fail(MPH_ERR_STATE, "neighbour row length mismatch");
-        for (int j : row) ind[w++] = j;
     }
     return MPH_OK;
 }
 
-extern "C" int mph_get_stats(const mph_ctx* c, mph_stats* out)
+extern "C" int mph_get_stats(const mph_ctx* cc, mph_stats* out)
 {
+    mph_ctx* c = const_cast<mph_ctx*>(cc);
     if (!c || !out) return fail(MPH_ERR_ARG, "mph_get_stats: bad argument");
     HIP_TRY(hipStreamSynchronize(c->stream));
     std::vector<long long> rb(c->ncases);
-    std::vector<int> cnt(c->n);
+    std::vector<int> len(c->nslots), cnt(c->ncases);
     int mx = 0;
     HIP_TRY(hipMemcpy(rb.data(), c->rebuilds.p, sizeof(long long) * c->ncases, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(cnt.data(), c->ncnt.p, sizeof(int) * c->n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(len.data(), c->cl_len.p, sizeof(int) * c->nslots, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cnt.data(), c->cl_count.p, sizeof(int) * c->ncases, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(&mx, c->max_len.p, sizeof(int), hipMemcpyDeviceToHost));
     out->steps = c->steps;
     out->rebuilds = 0;
     for (long long r : rb) out->rebuilds = std::max(out->rebuilds, r);
     out->list_entries = 0;
-    for (int v : cnt) out->list_entries += v;
+    for (int k = 0; k < c->ncases; ++k)
+        for (int g = 0; g < cnt[k]; ++g) out->list_entries += len[c->cases[k].k.cl_base + g];
     out->list_width = c->width;
     out->max_list_len = mx;
-    out->sweep_a_min_waves = kSweepMinWaves[c->variant_a];
-    out->sweep_b_min_waves = kSweepMinWaves[c->variant_b];
+    out->cluster_size = c->K;
+    out->clusters = 0;
+    for (int v : cnt) out->clusters += v;
     return MPH_OK;
 }
 

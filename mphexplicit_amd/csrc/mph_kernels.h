@@ -8,8 +8,7 @@
 //   aux   double4 {Gx, Gy, Gz, PA}  gravity centre and attractive pressure
 //   force double4 {Fx, Fy, Fz, 0}
 //   snap  double4                   positions at the last list build
-//   nbr   int [width/4][stride][4]  ELL neighbour lists in groups of four: entry k of particle i
-//                                   at ((k/4)*stride + i)*4 + k%4, one 16-byte load per group
+//   cl_nbr int [clusters][width]    union neighbour list of each cluster (see "Cluster tiles")
 //
 // Every particle kernel is launched over a block map: block b handles particles
 // [map[b].y, min(map[b].y + blockDim, case.end)) of case map[b].x, so the case constants
@@ -35,7 +34,8 @@ constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
-constexpr int kXcds = 8;   // MI355X: 8 XCDs x 32 CUs, workgroups dispatched round-robin
+constexpr int kXcds = 8;
+constexpr int kSubBits = 6;  // sub-cell bits of a sort key   // MI355X: 8 XCDs x 32 CUs, workgroups dispatched round-robin
 
 __device__ __forceinline__ int tag_of(double w) { return (int)__double_as_longlong(w); }
 
@@ -154,18 +154,34 @@ k_move(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
 // -------------------------------------------------------------------------------------
 // bucketing: cell key, LSD radix sort, cell starts, reorder (main.cpp:1111-1326)
 
-// CellId of main.cpp:96-100 (the 3D form; 2D cases have cc[2] == 1) -> storage key
-__device__ __forceinline__ int cell_key(const CaseConst& C, const int* __restrict__ rank,
-                                        const double4& p, int type)
+// CellId of main.cpp:96-100 (the 3D form; 2D cases have cc[2] == 1) -> storage key.
+// The key is (cell key << kSubBits) | sub-cell, where the cell key is
+// key_base + 2*rank(cell) + (wall ? 1 : 0) and the sub-cell code orders the particles of a
+// cell in 4x4x4 Morton bins, so that consecutive particles of a cell are close in space.
+__device__ __forceinline__ int sub_bin(double t)
 {
-    int ix = ((int)floor((p.x - C.dmin[0]) / C.cw[0])) % C.cc[0];
-    int iy = ((int)floor((p.y - C.dmin[1]) / C.cw[1])) % C.cc[1];
-    int iz = ((int)floor((p.z - C.dmin[2]) / C.cw[2])) % C.cc[2];
+    const double f = (t - floor(t)) * 4.0;
+    const int b = (int)f;
+    return b < 0 ? 0 : (b > 3 ? 3 : b);
+}
+
+__device__ __forceinline__ unsigned cell_key(const CaseConst& C, const int* __restrict__ rank,
+                                             const double4& p, int type)
+{
+    const double tx = (p.x - C.dmin[0]) / C.cw[0];
+    const double ty = (p.y - C.dmin[1]) / C.cw[1];
+    const double tz = (p.z - C.dmin[2]) / C.cw[2];
+    int ix = ((int)floor(tx)) % C.cc[0];
+    int iy = ((int)floor(ty)) % C.cc[1];
+    int iz = ((int)floor(tz)) % C.cc[2];
     ix = (ix % C.cc[0] + C.cc[0]) % C.cc[0];
     iy = (iy % C.cc[1] + C.cc[1]) % C.cc[1];
     iz = (iz % C.cc[2] + C.cc[2]) % C.cc[2];
     const int id = (ix * C.cc[1] + iy) * C.cc[2] + iz;
-    return C.key_base + 2 * rank[C.rank_base + id] + (type >= 2 ? 1 : 0);
+    const int sx = sub_bin(tx), sy = sub_bin(ty), sz = sub_bin(tz);
+    const unsigned sub = (sx & 1) | (sy & 1) << 1 | (sz & 1) << 2 | (sx >> 1) << 3 | (sy >> 1) << 4 | (sz >> 1) << 5;
+    const unsigned ck = (unsigned)(C.key_base + 2 * rank[C.rank_base + id] + (type >= 2 ? 1 : 0));
+    return ck << kSubBits | sub;
 }
 
 // keys of rebuilding cases from positions; other cases keep their sorted keys
@@ -184,7 +200,7 @@ k_keys(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
     unsigned key;
     if (flags[m.x]) {
         const double4 p = pos[i];
-        key = (unsigned)cell_key(C, rank, p, tag_of(p.w) & 3);
+        key = cell_key(C, rank, p, tag_of(p.w) & 3);
     } else {
         key = kprev[i];
     }
@@ -283,7 +299,7 @@ k_radix_scatter(const unsigned* __restrict__ kin, const int* __restrict__ vin,
     }
 }
 
-// start[k] = first sorted slot with key >= k, for k in [0, nkeys]
+// start[k] = first sorted slot whose cell key is >= k, for cell keys k in [0, nkeys]
 __global__ void __launch_bounds__(256)
 k_cell_start(const unsigned* __restrict__ keys, int n, int nkeys, int* __restrict__ start,
              const int* __restrict__ any_flag)
@@ -291,10 +307,11 @@ k_cell_start(const unsigned* __restrict__ keys, int n, int nkeys, int* __restric
     if (*any_flag == 0) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k > nkeys) return;
+    const unsigned kk = (unsigned)k << kSubBits;
     int lo = 0, hi = n;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (keys[mid] < (unsigned)k) lo = mid + 1; else hi = mid;
+        if (keys[mid] < kk) lo = mid + 1; else hi = mid;
     }
     start[k] = lo;
 }
@@ -341,22 +358,26 @@ k_scatter_back(const CaseConst* __restrict__ cases, const int2* __restrict__ bma
 }
 
 // -------------------------------------------------------------------------------------
-// neighbour lists (main.cpp:1329-1596): for each particle the fluid candidates of the
-// 3^d periodic cells, then (unless it is a wall particle with no fluid neighbour,
-// main.cpp:1416) the wall candidates; kept if q^2 <= (MaxRadius+MARGIN)^2, with the
-// reference's exact periodic difference.  Cells are visited in ascending reference cell
-// id, as the reference fills its lists.  Self is not stored; count_out carries the
-// reference's NeighborCount (list length including self).
+// Cluster tiles.  After each rebuild the particles of every cell (fluid then wall, in
+// sub-cell Morton order) are cut into clusters of K consecutive particles.  A cluster
+// shares the cell's 3^d stencil, and its list is the union, in ascending sorted index, of
+// the exact neighbour lists of its members (the reference's lists, main.cpp:1329-1596:
+// q^2 <= (MaxRadius+MARGIN)^2 with the periodic difference of main.cpp:1398, and no list
+// for a wall particle without fluid neighbours, main.cpp:1416).
+//
+// A wavefront evaluates one cluster: lane = row * S + slot with K rows (the members) and
+// S = 64/K slots.  At each step the S slots take S consecutive union entries, so one
+// 16-byte load per lane fetches S distinct partners and each is used by K rows; the
+// per-member sums are reduced across the S slot lanes with shuffles at the end.  Pairs
+// outside a member's cut-offs contribute exactly zero (the kernel factors vanish there),
+// so each member's sums cover exactly the pairs of its reference list that interact.
+//
+// Cluster table entry: int4 {first particle, size, reference cell id, case}.
 
-__device__ __forceinline__ size_t ell_slot(int k, int stride, int i)
-{
-    return ((size_t)(k >> 2) * stride + i) * 4 + (k & 3);
-}
-
+// the periodic window {ic-1, ic, ic+1} (or {ic} for range 0) in ascending order,
+// repeats included when the axis has fewer than 3 cells (as main.cpp:1365-1391)
 __device__ __forceinline__ int axis_window(int ic, int c, int range, int* out)
 {
-    // the periodic window {ic-1, ic, ic+1} (or {ic} for range 0) in ascending order,
-    // repeats included when the axis has fewer than 3 cells (as main.cpp:1365-1391)
     if (range == 0) { out[0] = ic; return 1; }
     int a = (ic - 1 + c) % c, b = ic, d = (ic + 1) % c;
     if (a > b) { int t = a; a = b; b = t; }
@@ -366,114 +387,224 @@ __device__ __forceinline__ int axis_window(int ic, int c, int range, int* out)
     return 3;
 }
 
+// ranks of the stencil cells of reference cell `cid`, ascending (storage order)
+__device__ __forceinline__ int stencil_ranks(const CaseConst& C, const int* __restrict__ rank, int cid,
+                                             int* rk)
+{
+    const int iz = cid % C.cc[2];
+    const int iy = (cid / C.cc[2]) % C.cc[1];
+    const int ix = cid / (C.cc[1] * C.cc[2]);
+    int wx[3], wy[3], wz[3];
+    const int nx = axis_window(ix, C.cc[0], 1, wx);
+    const int ny = axis_window(iy, C.cc[1], 1, wy);
+    const int nz = axis_window(iz, C.cc[2], C.dim == 2 ? 0 : 1, wz);
+    int n = 0;
+    for (int a = 0; a < nx; ++a)
+        for (int b = 0; b < ny; ++b)
+            for (int c = 0; c < nz; ++c) {
+                int v = rank[C.rank_base + (wx[a] * C.cc[1] + wy[b]) * C.cc[2] + wz[c]];
+                int q = n++;
+                while (q > 0 && rk[q - 1] > v) { rk[q] = rk[q - 1]; --q; }
+                rk[q] = v;
+            }
+    return n;
+}
+
+// clusters per cell: ceil(particles / K), for every cell slot (storage order)
 __global__ void __launch_bounds__(256)
-k_build_lists(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
-              const int* __restrict__ flags, const int* __restrict__ any_flag,
-              const double4* __restrict__ pos, const unsigned* __restrict__ keys,
-              const int* __restrict__ start, const int* __restrict__ rank,
-              const int* __restrict__ cell_of_rank, int* __restrict__ nbr, int stride, int width,
-              int* __restrict__ ncnt, int* __restrict__ count_out, int* __restrict__ max_len)
+k_cluster_count(const CaseConst* __restrict__ cases, const int* __restrict__ cell_case, int ncells,
+                const int* __restrict__ start, int K, unsigned* __restrict__ ccount,
+                const int* __restrict__ any_flag)
 {
     if (*any_flag == 0) return;
-    const int2 m = bmap[blockIdx.x];
-    if (m.x < 0) return;
-    const CaseConst& C = cases[m.x];
-    if (flags[m.x] == 0) return;          // uniform: a workgroup belongs to one case
-    const int i = m.y + (int)threadIdx.x;
-    int count = 0;                        // stored entries (self excluded)
-    if (i < C.end) {
-        const double4 pi = pos[i];
-        const int kk = (int)keys[i] - C.key_base;
-        const bool wall = kk & 1;
-        const int cid = cell_of_rank[C.rank_base + (kk >> 1)];
-        const int iz = cid % C.cc[2];
-        const int iy = (cid / C.cc[2]) % C.cc[1];
-        const int ix = cid / (C.cc[1] * C.cc[2]);
-        int wx[3], wy[3], wz[3];
-        const int nx = axis_window(ix, C.cc[0], 1, wx);
-        const int ny = axis_window(iy, C.cc[1], 1, wy);
-        const int nz = axis_window(iz, C.cc[2], C.dim == 2 ? 0 : 1, wz);
-        int seen_self = 0;
-        for (int pass = 0; pass < 2; ++pass) {
-            if (pass == 1 && wall && count == 0) break;
-            for (int a = 0; a < nx; ++a)
-                for (int b = 0; b < ny; ++b)
-                    for (int c = 0; c < nz; ++c) {
-                        const int cell = (wx[a] * C.cc[1] + wy[b]) * C.cc[2] + wz[c];
-                        const int k0 = C.key_base + 2 * rank[C.rank_base + cell] + pass;
-                        const int lo = start[k0], hi = start[k0 + 1];
-                        for (int j = lo; j < hi; ++j) {
-                            const double4 pj = pos[j];
-                            double dx, dy, dz;
-                            rel3(C, pi, pj, dx, dy, dz);
-                            const double q2 = dx * dx + dy * dy + dz * dz;
-                            if (q2 <= C.cut2) {
-                                if (j == i) { seen_self = 1; continue; }
-                                if (count < width) nbr[ell_slot(count, stride, i)] = j;
-                                ++count;
-                            }
-                        }
-                    }
-        }
-        ncnt[i] = count < width ? count : width;
-        count_out[i] = count + seen_self;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ncells) return;
+    const CaseConst& C = cases[cell_case[s]];
+    const int k0 = C.key_base + 2 * (s - C.rank_base);
+    const int n = start[k0 + 2] - start[k0];
+    ccount[s] = (unsigned)((n + K - 1) / K);
+}
+
+// cluster table from the scanned counts; non-rebuilt cases are rewritten with the same values
+__global__ void __launch_bounds__(256)
+k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell_case, int ncells,
+               const int* __restrict__ start, const int* __restrict__ cell_of_rank, int K,
+               const unsigned* __restrict__ coff, int4* __restrict__ cl_info, int* __restrict__ cl_count,
+               const int* __restrict__ any_flag)
+{
+    if (*any_flag == 0) return;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ncells) return;
+    const int c = cell_case[s];
+    const CaseConst& C = cases[c];
+    const int r = s - C.rank_base;
+    const int k0 = C.key_base + 2 * r;
+    const int first = start[k0], n = start[k0 + 2] - first;
+    const int base = (int)(coff[s] - coff[C.rank_base]);
+    const int nq = (n + K - 1) / K;
+    const int cid = cell_of_rank[s];
+    for (int q = 0; q < nq; ++q) {
+        const int g = base + q;
+        if (g < C.cl_cap) cl_info[C.cl_base + g] = make_int4(first + q * K, min(K, n - q * K), cid, c);
     }
-    int mx = count;
+    if (r == C.tcc - 1) cl_count[c] = base + nq;
+}
+
+// union lists: thread (cluster slot, member); the K member lanes of a cluster walk the
+// same candidates in lockstep and vote with a ballot
+__global__ void __launch_bounds__(256)
+k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slot_case, int nslots,
+                const int* __restrict__ flags, const int* __restrict__ any_flag,
+                const double4* __restrict__ pos, const int* __restrict__ start,
+                const int* __restrict__ rank, const int4* __restrict__ cl_info,
+                const int* __restrict__ cl_count, int K, int width, int* __restrict__ cl_nbr,
+                int* __restrict__ cl_len, int* __restrict__ count_out, int* __restrict__ max_len)
+{
+    if (*any_flag == 0) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = (int)(t / K), m = (int)(t % K);
+    int ulen = 0;
+    bool run = g < nslots;
+    int c = 0;
+    if (run) {
+        c = slot_case[g];
+        run = flags[c] != 0 && g - cases[c].cl_base < cl_count[c];
+    }
+    if (run) {
+        const CaseConst& C = cases[c];
+        const int4 info = cl_info[g];
+        const bool mine = m < info.y;
+        const int i = info.x + (mine ? m : 0);
+        const double4 pi = pos[i];
+        const bool wall_i = (tag_of(pi.w) & 3) >= 2;
+        int rk[27];
+        const int nc = stencil_ranks(C, rank, info.z, rk);
+        const int lane = threadIdx.x & 63;
+        const int seg = lane / K * K;
+        const unsigned long long segmask = (K == 64 ? ~0ull : ((1ull << K) - 1ull)) << seg;
+        int count = 0, nfluid = 0, self = 0;
+        int* out = cl_nbr + (size_t)g * width;
+        for (int q = 0; q < nc; ++q) {
+            const int k0 = C.key_base + 2 * rk[q];
+            const int lo = start[k0], mid = start[k0 + 1], hi = start[k0 + 2];
+            for (int j = lo; j < hi; ++j) {
+                const double4 pj = pos[j];
+                double dx, dy, dz;
+                rel3(C, pi, pj, dx, dy, dz);
+                const double q2 = dx * dx + dy * dy + dz * dz;
+                const bool in = mine && q2 <= C.cut2;
+                const bool other = in && j != i;
+                self |= in && j == i;
+                count += other;
+                nfluid += other && j < mid;
+                const unsigned long long vote = __ballot(other) & segmask;
+                if (vote) {
+                    if (m == 0 && ulen < width) out[ulen] = j;
+                    ++ulen;
+                }
+            }
+        }
+        if (mine) count_out[i] = (wall_i && nfluid == 0) ? 0 : count + self;
+        if (m == 0) cl_len[g] = ulen < width ? ulen : width;
+    }
+    int mx = ulen;
     for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
     if ((threadIdx.x & 63) == 0) atomicMax(max_len, mx);
 }
 
 // -------------------------------------------------------------------------------------
-// Partner loop shared by the sweeps.  List entries arrive four at a time (one 16-byte
-// load per group); the data of the next partner is loaded while the current one is
-// evaluated, alternating two register sets, so a wavefront always has loads in flight.
-// Entries past a particle's count are never dereferenced.
+// Cluster frame of a wavefront in the sweep kernels: which cluster, which row/slot, and
+// the software-pipelined walk over the union list (indices two steps ahead, partner data
+// one step ahead in alternating register sets).
 
-template <class Load, class Eval>
-__device__ __forceinline__ void for_partners(const int* __restrict__ nbr, int stride, int i, int cnt,
-                                             Load load, Eval eval)
+template <int K>
+struct Frame {
+    static constexpr int S = 64 / K;
+    int g;          // global cluster slot
+    int len;        // union length
+    int row, slot;
+    int i;          // member particle of this lane's row
+    bool valid;     // row < cluster size
+    const int* list;
+};
+
+template <int K>
+__device__ __forceinline__ bool cluster_frame(const CaseConst& C, int first_slot, const int4* __restrict__ cl_info,
+                                              const int* __restrict__ cl_count, int case_id,
+                                              const int* __restrict__ cl_nbr, const int* __restrict__ cl_len,
+                                              int width, Frame<K>& f)
 {
-    const int4* __restrict__ e4 = reinterpret_cast<const int4*>(nbr);
-    const int ng = (cnt + 3) >> 2;
-    int4 cur = ng > 0 ? e4[i] : make_int4(i, i, i, i);
-    auto A = load(cnt > 0 ? cur.x : i);
-    for (int g = 0; g < ng; ++g) {
-        const int k = 4 * g;
-        const int4 nxt = g + 1 < ng ? e4[(size_t)(g + 1) * stride + i] : make_int4(i, i, i, i);
-        auto B = load(k + 1 < cnt ? cur.y : i);
-        eval(A);                                   // k < cnt holds inside the loop
-        A = load(k + 2 < cnt ? cur.z : i);
-        if (k + 1 < cnt) eval(B);
-        B = load(k + 3 < cnt ? cur.w : i);
-        if (k + 2 < cnt) eval(A);
-        A = load(k + 4 < cnt ? nxt.x : i);
-        if (k + 3 < cnt) eval(B);
-        cur = nxt;
+    const int local = first_slot + (int)(threadIdx.x >> 6);
+    if (local >= cl_count[case_id]) return false;        // wave-uniform
+    f.g = C.cl_base + local;
+    const int4 info = cl_info[f.g];
+    const int lane = threadIdx.x & 63;
+    f.row = lane / Frame<K>::S;
+    f.slot = lane % Frame<K>::S;
+    f.valid = f.row < info.y;
+    f.i = info.x + (f.valid ? f.row : 0);
+    f.len = cl_len[f.g];
+    f.list = cl_nbr + (size_t)f.g * width;
+    return true;
+}
+
+template <int K, class Load, class Eval>
+__device__ __forceinline__ void walk_union(const Frame<K>& f, Load load, Eval eval)
+{
+    constexpr int S = Frame<K>::S;
+    auto idx = [&](int step) { const int e = step * S + f.slot; return e < f.len ? f.list[e] : -1; };
+    const int steps = (f.len + S - 1) / S;
+    int j0 = idx(0), j1 = idx(1);
+    auto A = load(j0 >= 0 ? j0 : f.i);
+    for (int k = 0; k < steps; k += 2) {
+        const int j2 = idx(k + 2);
+        auto B = load(j1 >= 0 ? j1 : f.i);
+        if (j0 >= 0 && j0 != f.i && f.valid) eval(A);
+        const int j3 = idx(k + 3);
+        A = load(j2 >= 0 ? j2 : f.i);
+        if (j1 >= 0 && j1 != f.i && f.valid) eval(B);
+        j0 = j2;
+        j1 = j3;
     }
+}
+
+template <int K>
+__device__ __forceinline__ double slot_sum(double v)
+{
+    for (int off = Frame<K>::S / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, Frame<K>::S);
+    return v;
+}
+
+template <int K>
+__device__ __forceinline__ int slot_sum_i(int v)
+{
+    for (int off = Frame<K>::S / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, Frame<K>::S);
+    return v;
+}
+
+// true if no list partner of a particle at p can need a periodic image on any axis
+__device__ __forceinline__ bool away(const CaseConst& C, const double4& p)
+{
+    return away_from_faces(C, p);
 }
 
 // -------------------------------------------------------------------------------------
 // sweep A: number densities, gravity centre, volumetric strain, divergence
 // (main.cpp:1665-1727, 1831-1896) with the pointwise coefficients and pressures of
-// main.cpp:1631-1662, 1898-1906, 1729-1740 fused into the epilogue.
-//
-// Per pair: exact periodic difference (the division-free forms above), q^2, one
-// reciprocal square root, then the kernel polynomials with their constant prefactors
-// pulled out of the sums.  The kernels are evaluated branch-free with the factor
-// max(1 - r/h, 0), which is zero outside the support exactly as the reference's cut-offs.
+// main.cpp:1631-1662, 1898-1906, 1729-1740 fused into the epilogue.  Per pair: exact
+// periodic difference, q^2, one reciprocal square root, kernel polynomials with their
+// constant prefactors pulled out of the sums, branch-free with max(1 - r/h, 0).
 
 struct PairA {
     double4 p;
     double vx, vy, vz;
 };
 
-template <int SURF, bool WRAP>
-__device__ __forceinline__ void sweep_a_loop(const CaseConst& C, const double* tab, int ti,
-                                             const double4& pi, double vix, double viy, double viz,
-                                             const double4* __restrict__ pos, const double4* vel,
-                                             const int* __restrict__ nbr, int stride, int i, int cnt,
-                                             double& na, double& gx, double& gy, double& gz,
-                                             double& wsum, double& dsum, int& npair)
+template <int SURF, int K, bool WRAP>
+__device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
+                                              const double4& pi, double vix, double viy, double viz,
+                                              const double4* __restrict__ pos, const double4* vel,
+                                              double* acc, int& npair)
 {
     auto load = [&](int j) {
         PairA q;
@@ -495,26 +626,28 @@ __device__ __forceinline__ void sweep_a_loop(const CaseConst& C, const double* t
         const double oma = fmax(1.0 - sa, 0.0);
         const double ratio = C.ratio_one ? 1.0 : tab[16 + ti * 4 + (tag_of(q.p.w) & 3)];
         const double wa = ratio * (oma * oma);
-        na = fma(wa, sa, na);
+        acc[0] = fma(wa, sa, acc[0]);
         if (SURF) {
-            gx = fma(dx, wa, gx);
-            gy = fma(dy, wa, gy);
-            gz = fma(dz, wa, gz);
+            acc[1] = fma(dx, wa, acc[1]);
+            acc[2] = fma(dy, wa, acc[2]);
+            acc[3] = fma(dz, wa, acc[3]);
         }
         const double omp = fmax(fma(-r, C.inv_hp, 1.0), 0.0);
-        wsum = fma(omp, omp, wsum);
+        acc[4] = fma(omp, omp, acc[4]);
         const double ud = fma(q.vz - viz, dz, fma(q.vy - viy, dy, (q.vx - vix) * dx));
-        dsum = fma(ud, omp * ir, dsum);
+        acc[5] = fma(ud, omp * ir, acc[5]);
     };
-    for_partners(nbr, stride, i, cnt, load, eval);
+    walk_union<K>(f, load, eval);
 }
 
-template <int SURF, int MINW>
-__global__ void __launch_bounds__(256, MINW)
-k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
+template <int SURF, int K>
+__global__ void __launch_bounds__(256)
+k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
+          const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
+          const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
+          const int* __restrict__ count_out,
           const double4* __restrict__ pos, double4* vel, double4* __restrict__ aux,
           double* __restrict__ dens_a, double* __restrict__ vstrain, double* __restrict__ divp,
-          const int* __restrict__ nbr, const int* __restrict__ ncnt, int stride,
           double* __restrict__ wallc, int ncases, unsigned long long* __restrict__ pair_count)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -525,7 +658,7 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
                 for (int d = 0; d < 3; ++d)
                     wallc[(c * kTypes + t) * 3 + d] += cases[c].wvel[t][d] * cases[c].dt;
     }
-    const int2 m = bmap[blockIdx.x];
+    const int2 m = cmap[blockIdx.x];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
     __shared__ double tab[32];                // [0,16): mu_ij, [16,32): InteractionRatio
@@ -534,33 +667,39 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
         tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
     }
     __syncthreads();
-    const int i = m.y + (int)threadIdx.x;
+    Frame<K> f;
+    if (!cluster_frame<K>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f)) return;
+    const double4 pi = pos[f.i];
+    const int ti = tag_of(pi.w) & 3;
+    const double vix = vel[f.i].x, viy = vel[f.i].y, viz = vel[f.i].z;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int npair = 0;
-    if (i < C.end) {
-        const double4 pi = pos[i];
-        const int ti = tag_of(pi.w) & 3;
-        const double vix = vel[i].x, viy = vel[i].y, viz = vel[i].z;
-        double na = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, wsum = 0.0, dsum = 0.0;
-        const int cnt = ncnt[i];
-        if (away_from_faces(C, pi))
-            sweep_a_loop<SURF, false>(C, tab, ti, pi, vix, viy, viz, pos, vel, nbr, stride, i, cnt,
-                                      na, gx, gy, gz, wsum, dsum, npair);
-        else
-            sweep_a_loop<SURF, true>(C, tab, ti, pi, vix, viy, viz, pos, vel, nbr, stride, i, cnt,
-                                     na, gx, gy, gz, wsum, dsum, npair);
-        na *= C.ca;
-        const double ep = C.cp * wsum - C.n0p;
-        const double dv = -C.dcp * dsum;
+    if (away(C, pi))
+        sweep_a_pairs<SURF, K, false>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, acc, npair);
+    else
+        sweep_a_pairs<SURF, K, true>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, acc, npair);
+    for (int q = 0; q < 6; ++q)
+        if (SURF || (q != 1 && q != 2 && q != 3)) acc[q] = slot_sum<K>(acc[q]);
+    npair = slot_sum_i<K>(npair);
+    if (f.slot == 0 && f.valid) {
+        const int i = f.i;
+        if (count_out[i] == 0)                 // no list (main.cpp:1416): all sums are zero
+            for (int q = 0; q < 6; ++q) acc[q] = 0.0;
+        const double na = acc[0] * C.ca;
+        const double ep = C.cp * acc[4] - C.n0p;
+        const double dv = -C.dcp * acc[5];
         const double kap = ep < 0.0 ? 0.0 : C.kbulk[ti];
         const double p = -C.lbulk[ti] * dv + kap * ep;
         double pa = C.cofa[ti] * (na - C.n0a) / C.l0;
         if (C.n0a <= na) pa = 0.0;
         const double gs = C.cg * C.g_scale;
         vel[i].w = p;
-        aux[i] = make_double4(gx * gs, gy * gs, gz * gs, pa);
+        aux[i] = make_double4(acc[1] * gs, acc[2] * gs, acc[3] * gs, pa);
         dens_a[i] = na;
         vstrain[i] = ep;
         divp[i] = dv;
+    } else {
+        npair = 0;
     }
     for (int off = 32; off > 0; off >>= 1) npair += __shfl_xor(npair, off);
     if ((threadIdx.x & 63) == 0 && npair) atomicAdd(pair_count, (unsigned long long)npair);
@@ -576,13 +715,11 @@ struct PairB {
     double4 p, v, a;
 };
 
-template <int SURF, bool WRAP>
-__device__ __forceinline__ void sweep_b_loop(const CaseConst& C, const double* tab, int ti,
-                                             const double4& pi, const double4& vi, const double4& ai,
-                                             const double4* __restrict__ pos, const double4* __restrict__ vel,
-                                             const double4* __restrict__ aux, const int* __restrict__ nbr,
-                                             int stride, int i, int cnt, double3& fp, double3& fv,
-                                             double3& fa, double3& fs)
+template <int SURF, int K, bool WRAP>
+__device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
+                                              const double4& pi, const double4& vi, const double4& ai,
+                                              const double4* __restrict__ pos, const double4* __restrict__ vel,
+                                              const double4* __restrict__ aux, double* acc)
 {
     const double mu_i = tab[ti * 4 + ti];
     auto load = [&](int j) {
@@ -602,25 +739,25 @@ __device__ __forceinline__ void sweep_b_loop(const CaseConst& C, const double* t
         const int tj = tag_of(q.p.w) & 3;
         // pressure
         const double cp = (vi.w + q.v.w) * fmax(fma(-r, C.inv_hp, 1.0), 0.0) * ir;
-        fp.x = fma(cp, dx, fp.x);
-        fp.y = fma(cp, dy, fp.y);
-        fp.z = fma(cp, dz, fp.z);
+        acc[0] = fma(cp, dx, acc[0]);
+        acc[1] = fma(cp, dy, acc[1]);
+        acc[2] = fma(cp, dz, acc[2]);
         // viscosity with the harmonic-mean coefficient of the type pair
         const double mu = C.mu_uniform ? mu_i : tab[ti * 4 + tj];
         const double ud = fma(q.v.z - vi.z, dz, fma(q.v.y - vi.y, dy, (q.v.x - vi.x) * dx));
         const double cv = mu * ud * fmax(fma(-r, C.inv_hv, 1.0), 0.0) * (ir * ir * ir);
-        fv.x = fma(cv, dx, fv.x);
-        fv.y = fma(cv, dy, fv.y);
-        fv.z = fma(cv, dz, fv.z);
+        acc[3] = fma(cv, dx, acc[3]);
+        acc[4] = fma(cv, dy, acc[4]);
+        acc[5] = fma(cv, dz, acc[5]);
         if (SURF) {
             const double rij = C.ratio_one ? 1.0 : tab[16 + ti * 4 + tj];
             const double rji = C.ratio_one ? 1.0 : tab[16 + tj * 4 + ti];
             const double s = r * C.inv_ha;
             const double om = fmax(1.0 - s, 0.0);
             const double ca = fma(q.a.w, rji, ai.w * rij) * om * fma(-3.0, s, 1.0) * ir;
-            fa.x = fma(ca, dx, fa.x);
-            fa.y = fma(ca, dy, fa.y);
-            fa.z = fma(ca, dz, fa.z);
+            acc[6] = fma(ca, dx, acc[6]);
+            acc[7] = fma(ca, dy, acc[7]);
+            acc[8] = fma(ca, dz, acc[8]);
             // diffuse interface with Q = R_ji G_j - R_ij G_i; a_j uses the type of i
             // (main.cpp:1792), so a_i factors out of the sum
             const double qx = fma(rji, q.a.x, -rij * ai.x);
@@ -628,21 +765,23 @@ __device__ __forceinline__ void sweep_b_loop(const CaseConst& C, const double* t
             const double qz = fma(rji, q.a.z, -rij * ai.z);
             const double qd = fma(qz, dz, fma(qy, dy, qx * dx)) * (C.dcg * ir);
             const double w = C.cg * om;
-            fs.x = fma(-om, fma(qd, dx, w * qx), fs.x);
-            fs.y = fma(-om, fma(qd, dy, w * qy), fs.y);
-            fs.z = fma(-om, fma(qd, dz, w * qz), fs.z);
+            acc[9] = fma(-om, fma(qd, dx, w * qx), acc[9]);
+            acc[10] = fma(-om, fma(qd, dy, w * qy), acc[10]);
+            acc[11] = fma(-om, fma(qd, dz, w * qz), acc[11]);
         }
     };
-    for_partners(nbr, stride, i, cnt, load, eval);
+    walk_union<K>(f, load, eval);
 }
 
-template <int SURF, int MINW>
-__global__ void __launch_bounds__(256, MINW)
-k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
+template <int SURF, int K>
+__global__ void __launch_bounds__(256)
+k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
+          const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
+          const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
+          const int* __restrict__ count_out,
           const double4* __restrict__ pos, const double4* __restrict__ vel,
           const double4* __restrict__ aux, double4* __restrict__ vnext,
-          double4* __restrict__ force, const int* __restrict__ nbr,
-          const int* __restrict__ ncnt, int stride, int* __restrict__ flags,
+          double4* __restrict__ force, int* __restrict__ flags,
           int* __restrict__ any_flag, long long* __restrict__ rebuilds, int ncases)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0 && ncases > 0) {
@@ -652,7 +791,7 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
         }
         *any_flag = 0;
     }
-    const int2 m = bmap[blockIdx.x];
+    const int2 m = cmap[blockIdx.x];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
     __shared__ double tab[32];
@@ -661,28 +800,34 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
         tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
     }
     __syncthreads();
-    const int i = m.y + (int)threadIdx.x;
-    if (i >= C.end) return;
-    const double4 pi = pos[i];
-    const double4 vi = vel[i];
+    Frame<K> f;
+    if (!cluster_frame<K>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f)) return;
+    const double4 pi = pos[f.i];
+    const double4 vi = vel[f.i];
     const int ti = tag_of(pi.w) & 3;
-    const double4 ai = SURF ? aux[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-    double3 fp = make_double3(0.0, 0.0, 0.0), fv = fp, fa = fp, fs = fp;
-    const int cnt = ncnt[i];
-    if (away_from_faces(C, pi))
-        sweep_b_loop<SURF, false>(C, tab, ti, pi, vi, ai, pos, vel, aux, nbr, stride, i, cnt, fp, fv, fa, fs);
+    const double4 ai = SURF ? aux[f.i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    constexpr int NA = SURF ? 12 : 6;
+    double acc[NA];
+    for (int q = 0; q < NA; ++q) acc[q] = 0.0;
+    if (away(C, pi))
+        sweep_b_pairs<SURF, K, false>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, acc);
     else
-        sweep_b_loop<SURF, true>(C, tab, ti, pi, vi, ai, pos, vel, aux, nbr, stride, i, cnt, fp, fv, fa, fs);
+        sweep_b_pairs<SURF, K, true>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, acc);
+    for (int q = 0; q < NA; ++q) acc[q] = slot_sum<K>(acc[q]);
+    if (f.slot != 0 || !f.valid) return;
+    const int i = f.i;
+    if (count_out[i] == 0)                     // no list (main.cpp:1416): all sums are zero
+        for (int q = 0; q < NA; ++q) acc[q] = 0.0;
     const double cp = C.dcp * C.vol;                      // pressure
     const double cv = C.visc_c * (-C.dcv) * C.vol;        // viscosity
-    double fx = fp.x * cp, fy = fp.y * cp, fz = fp.z * cp;
+    double fx = acc[0] * cp, fy = acc[1] * cp, fz = acc[2] * cp;
     if (SURF) {
         const double ca = C.dca * C.vol;
         const double cs = C.adiff[ti] * C.s_scale;
-        fx += fa.x * ca; fy += fa.y * ca; fz += fa.z * ca;
-        fx += fs.x * cs; fy += fs.y * cs; fz += fs.z * cs;
+        fx += acc[6] * ca; fy += acc[7] * ca; fz += acc[8] * ca;
+        fx += acc[9] * cs; fy += acc[10] * cs; fz += acc[11] * cs;
     }
-    fx += fv.x * cv; fy += fv.y * cv; fz += fv.z * cv;
+    fx += acc[3] * cv; fy += acc[4] * cv; fz += acc[5] * cv;
     double4 vn = make_double4(vi.x, vi.y, vi.z, 0.0);
     if (ti < 2) {
         const double ms = C.mass[ti];
@@ -704,32 +849,33 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
 //   pressure      P_i dw_P r
 //   attractive    PA_i R_ij dw_A r
 //   viscous       0.5 c_d mu_ij (u.e) (-dw_V)
-//   diffuse       a_i (G_i . x) R_ij w_G /l0 * (h_G/R2g)  and  a_i (G_i . x) R_ij dw_G r /l0 * (h_G/R2g)
+//   diffuse       a_i (G_i . x) R_ij w_G (h_G/R2g)/l0  and  a_i (G_i . x) R_ij dw_G r (h_G/R2g)/l0
 // with the reference's per-term cut-offs.  Velocities are those after the step's update.
 
+template <int K>
 __global__ void __launch_bounds__(256)
-k_virial(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
-         const double4* __restrict__ pos, const double4* __restrict__ vel,
-         const double4* __restrict__ vnext, const double4* __restrict__ aux,
-         const int* __restrict__ nbr, const int* __restrict__ ncnt, int stride,
-         double* __restrict__ vir)
+k_virial(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
+         const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
+         const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
+         const int* __restrict__ count_out, const double4* __restrict__ pos,
+         const double4* __restrict__ vel, const double4* __restrict__ vnext,
+         const double4* __restrict__ aux, double* __restrict__ vir)
 {
-    const int2 m = bmap[blockIdx.x];
+    const int2 m = cmap[blockIdx.x];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
-    const int i = m.y + (int)threadIdx.x;
-    if (i >= C.end) return;
-    const double4 pi = pos[i];
-    const double4 vi = vnext[i];
-    const double P = vel[i].w;
-    const double4 ai = aux[i];
+    Frame<K> f;
+    if (!cluster_frame<K>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f)) return;
+    const double4 pi = pos[f.i];
+    const double4 vi = vnext[f.i];
+    const double P = vel[f.i].w;
+    const double4 ai = aux[f.i];
     const int ti = tag_of(pi.w) & 3;
     const double a = C.adiff[ti];
     const double gsc = C.s_scale / C.vol;      // h_G / R2g / l0
     double tp = 0.0, ta = 0.0, tv = 0.0, ts = 0.0;
-    const int cnt = ncnt[i];
-    for (int k = 0; k < cnt; ++k) {
-        const int j = nbr[ell_slot(k, stride, i)];
+    auto load = [&](int j) { return j; };
+    auto eval = [&](int j) {
         const double4 pj = pos[j];
         double dx, dy, dz;
         rel3(C, pi, pj, dx, dy, dz);
@@ -751,9 +897,15 @@ k_virial(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
             const double ue = ((vj.x - vi.x) * dx + (vj.y - vi.y) * dy + (vj.z - vi.z) * dz) / r;
             tv += 0.5 * C.visc_c * C.muij[ti][tj] * ue * (-(C.dcv * (1.0 - r * C.inv_hv)));
         }
-    }
-    const double tr = ((tp + ta) + tv) + ts;
-    vir[i] = -tr / (C.dim == 2 ? 2.0 : 3.0);
+    };
+    walk_union<K>(f, load, eval);
+    tp = slot_sum<K>(tp);
+    ta = slot_sum<K>(ta);
+    tv = slot_sum<K>(tv);
+    ts = slot_sum<K>(ts);
+    if (f.slot != 0 || !f.valid) return;
+    const double tr = count_out[f.i] == 0 ? 0.0 : ((tp + ta) + tv) + ts;
+    vir[f.i] = -tr / (C.dim == 2 ? 2.0 : 3.0);
 }
 
 }  // namespace mph
