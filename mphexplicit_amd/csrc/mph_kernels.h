@@ -568,6 +568,49 @@ __device__ __forceinline__ void walk_union(const Frame<K>& f, Load load, Eval ev
     }
 }
 
+// The same walk with the partner data shared through LDS: only the S lanes of row 0 load
+// the partners of a step from global memory and store them to this wavefront's LDS slice
+// (two buffers); every row then reads its slot's partner from LDS (K lanes read each
+// address, a broadcast).  Global loads per pair drop by K; the next step's partners are
+// in flight while the current step is evaluated.
+template <int K, int NREC, class Fetch, class Eval>
+__device__ __forceinline__ void walk_union_lds(const Frame<K>& f, double4* __restrict__ lds, int* __restrict__ ldsj,
+                                               Fetch fetch, Eval eval)
+{
+    constexpr int S = Frame<K>::S;
+    const bool loader = f.row == 0;
+    const int steps = (f.len + S - 1) / S;
+    auto idx = [&](int step) { const int e = step * S + f.slot; return e < f.len ? f.list[e] : -1; };
+    double4 r[NREC];
+    int jr = -1, jn = -1;
+    if (loader) {
+        const int j0 = idx(0);
+        fetch(j0 >= 0 ? j0 : f.i, r);
+        for (int q = 0; q < NREC; ++q) lds[q * S + f.slot] = r[q];
+        ldsj[f.slot] = j0;
+        jr = idx(1);
+        if (steps > 1) fetch(jr >= 0 ? jr : f.i, r);
+        jn = idx(2);
+    }
+    for (int k = 0; k < steps; ++k) {
+        const int b = k & 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double4 q[NREC];
+        for (int t = 0; t < NREC; ++t) q[t] = lds[(b * NREC + t) * S + f.slot];
+        const int j = ldsj[b * S + f.slot];
+        if (loader && k + 1 < steps) {
+            for (int t = 0; t < NREC; ++t) lds[((b ^ 1) * NREC + t) * S + f.slot] = r[t];
+            ldsj[(b ^ 1) * S + f.slot] = jr;
+            jr = jn;
+            if (k + 2 < steps) fetch(jr >= 0 ? jr : f.i, r);
+            jn = idx(k + 3);
+        }
+        if (j >= 0 && j != f.i && f.valid) eval(q);
+    }
+}
+
 template <int K>
 __device__ __forceinline__ double slot_sum(double v)
 {
@@ -604,27 +647,25 @@ template <int SURF, int K, bool WRAP>
 __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, double vix, double viy, double viz,
                                               const double4* __restrict__ pos, const double4* vel,
-                                              double* acc, int& npair)
+                                              double4* lds, int* ldsj, double* acc, int& npair)
 {
-    auto load = [&](int j) {
-        PairA q;
-        q.p = pos[j];
-        q.vx = vel[j].x;          // P (vel.w) of partners is being written by this kernel
-        q.vy = vel[j].y;
-        q.vz = vel[j].z;
-        return q;
+    auto fetch = [&](int j, double4* r) {
+        r[0] = pos[j];
+        // P (vel.w) of partners is being written by this kernel: read x, y, z only
+        r[1] = make_double4(vel[j].x, vel[j].y, vel[j].z, 0.0);
     };
-    auto eval = [&](const PairA& q) {
-        const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(q.p.x, pi.x, C.hwid[0]);
-        const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(q.p.y, pi.y, C.hwid[1]);
-        const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(q.p.z, pi.z, C.hwid[2]);
+    auto eval = [&](const double4* q) {
+        const double4 p = q[0], v = q[1];
+        const double dx = WRAP ? rel_wrap(p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(p.x, pi.x, C.hwid[0]);
+        const double dy = WRAP ? rel_wrap(p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(p.y, pi.y, C.hwid[1]);
+        const double dz = WRAP ? rel_wrap(p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(p.z, pi.z, C.hwid[2]);
         const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
         npair += q2 <= C.hmax2;
         const double ir = rsqrt_nr(q2);
         const double r = q2 * ir;
         const double sa = r * C.inv_ha;
         const double oma = fmax(1.0 - sa, 0.0);
-        const double ratio = C.ratio_one ? 1.0 : tab[16 + ti * 4 + (tag_of(q.p.w) & 3)];
+        const double ratio = C.ratio_one ? 1.0 : tab[16 + ti * 4 + (tag_of(p.w) & 3)];
         const double wa = ratio * (oma * oma);
         acc[0] = fma(wa, sa, acc[0]);
         if (SURF) {
@@ -634,10 +675,10 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         }
         const double omp = fmax(fma(-r, C.inv_hp, 1.0), 0.0);
         acc[4] = fma(omp, omp, acc[4]);
-        const double ud = fma(q.vz - viz, dz, fma(q.vy - viy, dy, (q.vx - vix) * dx));
+        const double ud = fma(v.z - viz, dz, fma(v.y - viy, dy, (v.x - vix) * dx));
         acc[5] = fma(ud, omp * ir, acc[5]);
     };
-    walk_union<K>(f, load, eval);
+    walk_union_lds<K, 2>(f, lds, ldsj, fetch, eval);
 }
 
 template <int SURF, int K>
@@ -662,6 +703,9 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
     __shared__ double tab[32];                // [0,16): mu_ij, [16,32): InteractionRatio
+    constexpr int S = Frame<K>::S;
+    __shared__ double4 lds[4][2 * 2 * S];     // per wavefront: 2 buffers x {pos, vel} x S
+    __shared__ int ldsj[4][2 * S];
     if (threadIdx.x < 16) {
         tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
         tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
@@ -669,15 +713,17 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     __syncthreads();
     Frame<K> f;
     if (!cluster_frame<K>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f)) return;
+    const int w = threadIdx.x >> 6;
     const double4 pi = pos[f.i];
     const int ti = tag_of(pi.w) & 3;
     const double vix = vel[f.i].x, viy = vel[f.i].y, viz = vel[f.i].z;
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int npair = 0;
-    if (away(C, pi))
-        sweep_a_pairs<SURF, K, false>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, acc, npair);
+    // a wavefront takes the periodic path if any of its rows needs it (uniform branch)
+    if (__ballot(!away(C, pi)) == 0ull)
+        sweep_a_pairs<SURF, K, false>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w], acc, npair);
     else
-        sweep_a_pairs<SURF, K, true>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, acc, npair);
+        sweep_a_pairs<SURF, K, true>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w], acc, npair);
     for (int q = 0; q < 6; ++q)
         if (SURF || (q != 1 && q != 2 && q != 3)) acc[q] = slot_sum<K>(acc[q]);
     npair = slot_sum_i<K>(npair);
@@ -719,50 +765,50 @@ template <int SURF, int K, bool WRAP>
 __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, const double4& vi, const double4& ai,
                                               const double4* __restrict__ pos, const double4* __restrict__ vel,
-                                              const double4* __restrict__ aux, double* acc)
+                                              const double4* __restrict__ aux, double4* lds, int* ldsj, double* acc)
 {
     const double mu_i = tab[ti * 4 + ti];
-    auto load = [&](int j) {
-        PairB q;
-        q.p = pos[j];
-        q.v = vel[j];
-        if (SURF) q.a = aux[j];
-        return q;
+    auto fetch = [&](int j, double4* r) {
+        r[0] = pos[j];
+        r[1] = vel[j];
+        if (SURF) r[2] = aux[j];
     };
-    auto eval = [&](const PairB& q) {
-        const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(q.p.x, pi.x, C.hwid[0]);
-        const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(q.p.y, pi.y, C.hwid[1]);
-        const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(q.p.z, pi.z, C.hwid[2]);
+    auto eval = [&](const double4* q) {
+        const double4 p = q[0], v = q[1];
+        const double dx = WRAP ? rel_wrap(p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(p.x, pi.x, C.hwid[0]);
+        const double dy = WRAP ? rel_wrap(p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(p.y, pi.y, C.hwid[1]);
+        const double dz = WRAP ? rel_wrap(p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(p.z, pi.z, C.hwid[2]);
         const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
         const double ir = rsqrt_nr(q2);
         const double r = q2 * ir;
-        const int tj = tag_of(q.p.w) & 3;
+        const int tj = tag_of(p.w) & 3;
         // pressure
-        const double cp = (vi.w + q.v.w) * fmax(fma(-r, C.inv_hp, 1.0), 0.0) * ir;
+        const double cp = (vi.w + v.w) * fmax(fma(-r, C.inv_hp, 1.0), 0.0) * ir;
         acc[0] = fma(cp, dx, acc[0]);
         acc[1] = fma(cp, dy, acc[1]);
         acc[2] = fma(cp, dz, acc[2]);
         // viscosity with the harmonic-mean coefficient of the type pair
         const double mu = C.mu_uniform ? mu_i : tab[ti * 4 + tj];
-        const double ud = fma(q.v.z - vi.z, dz, fma(q.v.y - vi.y, dy, (q.v.x - vi.x) * dx));
+        const double ud = fma(v.z - vi.z, dz, fma(v.y - vi.y, dy, (v.x - vi.x) * dx));
         const double cv = mu * ud * fmax(fma(-r, C.inv_hv, 1.0), 0.0) * (ir * ir * ir);
         acc[3] = fma(cv, dx, acc[3]);
         acc[4] = fma(cv, dy, acc[4]);
         acc[5] = fma(cv, dz, acc[5]);
         if (SURF) {
+            const double4 a = q[SURF ? 2 : 0];
             const double rij = C.ratio_one ? 1.0 : tab[16 + ti * 4 + tj];
             const double rji = C.ratio_one ? 1.0 : tab[16 + tj * 4 + ti];
             const double s = r * C.inv_ha;
             const double om = fmax(1.0 - s, 0.0);
-            const double ca = fma(q.a.w, rji, ai.w * rij) * om * fma(-3.0, s, 1.0) * ir;
+            const double ca = fma(a.w, rji, ai.w * rij) * om * fma(-3.0, s, 1.0) * ir;
             acc[6] = fma(ca, dx, acc[6]);
             acc[7] = fma(ca, dy, acc[7]);
             acc[8] = fma(ca, dz, acc[8]);
             // diffuse interface with Q = R_ji G_j - R_ij G_i; a_j uses the type of i
             // (main.cpp:1792), so a_i factors out of the sum
-            const double qx = fma(rji, q.a.x, -rij * ai.x);
-            const double qy = fma(rji, q.a.y, -rij * ai.y);
-            const double qz = fma(rji, q.a.z, -rij * ai.z);
+            const double qx = fma(rji, a.x, -rij * ai.x);
+            const double qy = fma(rji, a.y, -rij * ai.y);
+            const double qz = fma(rji, a.z, -rij * ai.z);
             const double qd = fma(qz, dz, fma(qy, dy, qx * dx)) * (C.dcg * ir);
             const double w = C.cg * om;
             acc[9] = fma(-om, fma(qd, dx, w * qx), acc[9]);
@@ -770,7 +816,7 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
             acc[11] = fma(-om, fma(qd, dz, w * qz), acc[11]);
         }
     };
-    walk_union<K>(f, load, eval);
+    walk_union_lds<K, SURF ? 3 : 2>(f, lds, ldsj, fetch, eval);
 }
 
 template <int SURF, int K>
@@ -795,6 +841,10 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
     __shared__ double tab[32];
+    constexpr int S = Frame<K>::S;
+    constexpr int NREC = SURF ? 3 : 2;
+    __shared__ double4 lds[4][2 * NREC * S];  // per wavefront: 2 buffers x records x S
+    __shared__ int ldsj[4][2 * S];
     if (threadIdx.x < 16) {
         tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
         tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
@@ -802,6 +852,7 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     __syncthreads();
     Frame<K> f;
     if (!cluster_frame<K>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f)) return;
+    const int w = threadIdx.x >> 6;
     const double4 pi = pos[f.i];
     const double4 vi = vel[f.i];
     const int ti = tag_of(pi.w) & 3;
@@ -809,10 +860,10 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     constexpr int NA = SURF ? 12 : 6;
     double acc[NA];
     for (int q = 0; q < NA; ++q) acc[q] = 0.0;
-    if (away(C, pi))
-        sweep_b_pairs<SURF, K, false>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, acc);
+    if (__ballot(!away(C, pi)) == 0ull)
+        sweep_b_pairs<SURF, K, false>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], ldsj[w], acc);
     else
-        sweep_b_pairs<SURF, K, true>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, acc);
+        sweep_b_pairs<SURF, K, true>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], ldsj[w], acc);
     for (int q = 0; q < NA; ++q) acc[q] = slot_sum<K>(acc[q]);
     if (f.slot != 0 || !f.valid) return;
     const int i = f.i;

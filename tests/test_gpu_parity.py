@@ -185,3 +185,20 @@ def test_golden_fixtures(solver_cls, name):
             err = normwise(f[k][ids], ref)
         assert err <= FP64_TOL, (k, err)
     assert np.array_equal(f["neighbor_count"][ids], g["step1_ncount"])
+
+
+@pytest.mark.parametrize("K", ["2", "4", "8"])
+@pytest.mark.parametrize("name", ["twofluid3d", "movingwall2d"])
+def test_every_cluster_size(solver_cls, name, K, monkeypatch):
+    # each tile shape the autotuner can pick gives the reference's lists and fields
+    monkeypatch.setenv("MPH_TUNE", K)
+    d = DECKS[name]()
+    orc = Oracle(d, nthreads=8)
+    s = solver_cls.from_deck(d)
+    assert s.stats()["cluster_size"] == int(K)
+    orc.step(3)
+    s.step(3)
+    ptr, ind = orc.neighbors()
+    assert _gpu_rows(s) == oracle_lists_by_id(orc.fields(), ptr, ind)
+    _compare(s.fields(), by_id(orc.fields()), "%s K=%s step 3" % (name, K), names=_fields_for(d),
+             p_scale=_pressure_scale(d))
