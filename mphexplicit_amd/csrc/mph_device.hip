@@ -687,7 +687,8 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         (rc = c->rank.alloc(hrank.size())) || (rc = c->cell_of_rank.alloc(hcell.size())) ||
         (rc = c->cell_case.alloc(c->ncells)) || (rc = c->ccount.alloc(c->ncells)) ||
         (rc = c->cl_count.alloc(ncases)) || (rc = c->max_len.alloc(1)) ||
-        (rc = c->rebuilds.alloc(ncases)) || (rc = c->pairs.alloc(1)) || (rc = c->pairs_scratch.alloc(1)))
+        (rc = c->rebuilds.alloc(ncases)) || (rc = c->pairs.alloc(kPairSlots * kPairStride)) ||
+        (rc = c->pairs_scratch.alloc(kPairSlots * kPairStride)))
         return rc;
     hipStream_t s = c->stream;
     HIP_TRY(hipMemcpyAsync(c->d_bmap.p, c->bmap_host.data(), sizeof(int2) * c->bmap_host.size(), hipMemcpyHostToDevice, s));
@@ -702,7 +703,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     HIP_TRY(hipMemsetAsync(c->force.p, 0, sizeof(double4) * n, s));
     HIP_TRY(hipMemsetAsync(c->vir.p, 0, sizeof(double) * n, s));
     HIP_TRY(hipMemsetAsync(c->rebuilds.p, 0, sizeof(long long) * ncases, s));
-    HIP_TRY(hipMemsetAsync(c->pairs.p, 0, sizeof(unsigned long long), s));
+    HIP_TRY(hipMemsetAsync(c->pairs.p, 0, sizeof(unsigned long long) * kPairSlots * kPairStride, s));
     // first bucketing and lists (main.cpp:358-359), for the cluster size the autotuner
     // picks (it builds and times every size, the last build is then redone for the winner)
     int K = 4;
@@ -786,7 +787,7 @@ extern "C" int mph_step_timed(mph_ctx* c, int nsteps, mph_timing* out)
     std::vector<long long> rb0(c->ncases), rb1(c->ncases);
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemcpy(rb0.data(), c->rebuilds.p, sizeof(long long) * c->ncases, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemsetAsync(c->pairs.p, 0, sizeof(unsigned long long), c->stream));
+    HIP_TRY(hipMemsetAsync(c->pairs.p, 0, sizeof(unsigned long long) * kPairSlots * kPairStride, c->stream));
     std::vector<hipEvent_t> ev((size_t)nsteps * 5, nullptr);
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     int rc = MPH_OK;
@@ -815,8 +816,10 @@ extern "C" int mph_step_timed(mph_ctx* c, int nsteps, mph_timing* out)
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     if (rc) return rc;
     if (se != hipSuccess) return fail(MPH_ERR_DEVICE, "step failed: %s", hipGetErrorString(se));
+    std::vector<unsigned long long> slots((size_t)kPairSlots * kPairStride);
+    HIP_TRY(hipMemcpy(slots.data(), c->pairs.p, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost));
     unsigned long long pairs = 0;
-    HIP_TRY(hipMemcpy(&pairs, c->pairs.p, sizeof(pairs), hipMemcpyDeviceToHost));
+    for (int k = 0; k < kPairSlots; ++k) pairs += slots[(size_t)k * kPairStride];
     HIP_TRY(hipMemcpy(rb1.data(), c->rebuilds.p, sizeof(long long) * c->ncases, hipMemcpyDeviceToHost));
     out->pairs = (long long)pairs;
     for (int k = 0; k < c->ncases; ++k) out->rebuilds = std::max(out->rebuilds, rb1[k] - rb0[k]);

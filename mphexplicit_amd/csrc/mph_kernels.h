@@ -35,7 +35,11 @@ constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kXcds = 8;
-constexpr int kSubBits = 6;  // sub-cell bits of a sort key   // MI355X: 8 XCDs x 32 CUs, workgroups dispatched round-robin
+constexpr int kSubBits = 6;
+// pair counters: one atomic per wavefront would serialise on a single address, so
+// wavefronts add into kPairSlots counters 128 bytes apart, summed on the host
+constexpr int kPairSlots = 1024;
+constexpr int kPairStride = 16;  // sub-cell bits of a sort key   // MI355X: 8 XCDs x 32 CUs, workgroups dispatched round-robin
 
 __device__ __forceinline__ int tag_of(double w) { return (int)__double_as_longlong(w); }
 
@@ -568,46 +572,60 @@ __device__ __forceinline__ void walk_union(const Frame<K>& f, Load load, Eval ev
     }
 }
 
-// The same walk with the partner data shared through LDS: only the S lanes of row 0 load
-// the partners of a step from global memory and store them to this wavefront's LDS slice
-// (two buffers); every row then reads its slot's partner from LDS (K lanes read each
-// address, a broadcast).  Global loads per pair drop by K; the next step's partners are
-// in flight while the current step is evaluated.
+// native 4-wide vectors: records built from them stay in registers (the HIP double4
+// wrapper type, copied as a whole, can keep a record in private memory)
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4d ld4(const double4* p) { return *reinterpret_cast<const v4d*>(p); }
+
+struct Rec {
+    v4d p, v, a;            // position+tag, velocity+P, gravity centre+PA (a only with surface tension)
+};
+
+// The same walk with the partner data staged through LDS in blocks of 64 union entries:
+// all 64 lanes of the wavefront load one entry each (ascending sorted indices, so the
+// loads of a block mostly share cache lines), store the records to this wavefront's LDS
+// slice, and then the K rows sweep the block in K steps of S slots, each row reading the
+// slot's record from LDS (K lanes read each address, a broadcast).  Every union entry is
+// fetched from global memory once per cluster instead of once per member, and the next
+// block's loads are in flight while the current block is evaluated.
 template <int K, int NREC, class Fetch, class Eval>
-__device__ __forceinline__ void walk_union_lds(const Frame<K>& f, double4* __restrict__ lds, int* __restrict__ ldsj,
+__device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v4d* __restrict__ lds, int* __restrict__ ldsj,
                                                Fetch fetch, Eval eval)
 {
     constexpr int S = Frame<K>::S;
-    const bool loader = f.row == 0;
-    const int steps = (f.len + S - 1) / S;
-    auto idx = [&](int step) { const int e = step * S + f.slot; return e < f.len ? f.list[e] : -1; };
-    double4 r[NREC];
-    int jr = -1, jn = -1;
-    if (loader) {
-        const int j0 = idx(0);
-        fetch(j0 >= 0 ? j0 : f.i, r);
-        for (int q = 0; q < NREC; ++q) lds[q * S + f.slot] = r[q];
-        ldsj[f.slot] = j0;
-        jr = idx(1);
-        if (steps > 1) fetch(jr >= 0 ? jr : f.i, r);
-        jn = idx(2);
-    }
-    for (int k = 0; k < steps; ++k) {
-        const int b = k & 1;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    const int lane = (int)(threadIdx.x & 63);
+    const int blocks = (f.len + 63) >> 6;
+    auto idx = [&](int blk) { const int e = (blk << 6) + lane; return e < f.len ? f.list[e] : -1; };
+    Rec r = {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}};
+    int jr = idx(0);
+    if (jr >= 0) fetch(jr, r);
+    for (int blk = 0; blk < blocks; ++blk) {
+        // LDS operations of one wavefront complete in issue order: the reads of the
+        // previous block are done before these stores land, and the stores are done
+        // before the reads below; the barriers keep the compiler from moving LDS
+        // accesses across the block boundary
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        double4 q[NREC];
-        for (int t = 0; t < NREC; ++t) q[t] = lds[(b * NREC + t) * S + f.slot];
-        const int j = ldsj[b * S + f.slot];
-        if (loader && k + 1 < steps) {
-            for (int t = 0; t < NREC; ++t) lds[((b ^ 1) * NREC + t) * S + f.slot] = r[t];
-            ldsj[(b ^ 1) * S + f.slot] = jr;
-            jr = jn;
-            if (k + 2 < steps) fetch(jr >= 0 ? jr : f.i, r);
-            jn = idx(k + 3);
+        lds[0 * 64 + lane] = r.p;
+        lds[1 * 64 + lane] = r.v;
+        if (NREC > 2) lds[2 * 64 + lane] = r.a;
+        ldsj[lane] = jr;
+        __builtin_amdgcn_wave_barrier();
+        if (blk + 1 < blocks) {
+            jr = idx(blk + 1);
+            if (jr >= 0) fetch(jr, r);
         }
-        if (j >= 0 && j != f.i && f.valid) eval(q);
+        const int rest = f.len - (blk << 6);
+        const int nsteps = rest >= 64 ? K : (rest + S - 1) / S;
+        for (int t = 0; t < nsteps; ++t) {
+            const int e = t * S + f.slot;
+            const int j = ldsj[e];
+            Rec q;
+            q.p = lds[0 * 64 + e];
+            q.v = lds[1 * 64 + e];
+            if (NREC > 2) q.a = lds[2 * 64 + e];
+            if (j >= 0 && j != f.i && f.valid) eval(q);
+        }
     }
 }
 
@@ -643,40 +661,45 @@ struct PairA {
     double vx, vy, vz;
 };
 
+struct AccA {
+    double na, gx, gy, gz, ws, ds;
+    int np;
+};
+
 template <int SURF, int K, bool WRAP>
 __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, double vix, double viy, double viz,
                                               const double4* __restrict__ pos, const double4* vel,
-                                              double4* lds, int* ldsj, double* acc, int& npair)
+                                              v4d* lds, int* ldsj, AccA& acc)
 {
-    auto fetch = [&](int j, double4* r) {
-        r[0] = pos[j];
+    auto fetch = [&](int j, Rec& r) {
+        r.p = ld4(pos + j);
         // P (vel.w) of partners is being written by this kernel: read x, y, z only
-        r[1] = make_double4(vel[j].x, vel[j].y, vel[j].z, 0.0);
+        const double4 u = vel[j];
+        r.v = v4d{u.x, u.y, u.z, 0.0};
     };
-    auto eval = [&](const double4* q) {
-        const double4 p = q[0], v = q[1];
-        const double dx = WRAP ? rel_wrap(p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(p.x, pi.x, C.hwid[0]);
-        const double dy = WRAP ? rel_wrap(p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(p.y, pi.y, C.hwid[1]);
-        const double dz = WRAP ? rel_wrap(p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(p.z, pi.z, C.hwid[2]);
+    auto eval = [&](const Rec& q) {
+        const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(q.p.x, pi.x, C.hwid[0]);
+        const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(q.p.y, pi.y, C.hwid[1]);
+        const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(q.p.z, pi.z, C.hwid[2]);
         const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        npair += q2 <= C.hmax2;
+        acc.np += q2 <= C.hmax2;
         const double ir = rsqrt_nr(q2);
         const double r = q2 * ir;
         const double sa = r * C.inv_ha;
         const double oma = fmax(1.0 - sa, 0.0);
-        const double ratio = C.ratio_one ? 1.0 : tab[16 + ti * 4 + (tag_of(p.w) & 3)];
+        const double ratio = C.ratio_one ? 1.0 : tab[16 + ti * 4 + (tag_of(q.p.w) & 3)];
         const double wa = ratio * (oma * oma);
-        acc[0] = fma(wa, sa, acc[0]);
+        acc.na = fma(wa, sa, acc.na);
         if (SURF) {
-            acc[1] = fma(dx, wa, acc[1]);
-            acc[2] = fma(dy, wa, acc[2]);
-            acc[3] = fma(dz, wa, acc[3]);
+            acc.gx = fma(dx, wa, acc.gx);
+            acc.gy = fma(dy, wa, acc.gy);
+            acc.gz = fma(dz, wa, acc.gz);
         }
         const double omp = fmax(fma(-r, C.inv_hp, 1.0), 0.0);
-        acc[4] = fma(omp, omp, acc[4]);
-        const double ud = fma(v.z - viz, dz, fma(v.y - viy, dy, (v.x - vix) * dx));
-        acc[5] = fma(ud, omp * ir, acc[5]);
+        acc.ws = fma(omp, omp, acc.ws);
+        const double ud = fma(q.v.z - viz, dz, fma(q.v.y - viy, dy, (q.v.x - vix) * dx));
+        acc.ds = fma(ud, omp * ir, acc.ds);
     };
     walk_union_lds<K, 2>(f, lds, ldsj, fetch, eval);
 }
@@ -704,8 +727,8 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     const CaseConst& C = cases[m.x];
     __shared__ double tab[32];                // [0,16): mu_ij, [16,32): InteractionRatio
     constexpr int S = Frame<K>::S;
-    __shared__ double4 lds[4][2 * 2 * S];     // per wavefront: 2 buffers x {pos, vel} x S
-    __shared__ int ldsj[4][2 * S];
+    __shared__ v4d lds[4][2 * 64];        // per wavefront: {pos, vel} x 64 entries
+    __shared__ int ldsj[4][64];
     if (threadIdx.x < 16) {
         tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
         tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
@@ -717,30 +740,35 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     const double4 pi = pos[f.i];
     const int ti = tag_of(pi.w) & 3;
     const double vix = vel[f.i].x, viy = vel[f.i].y, viz = vel[f.i].z;
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    int npair = 0;
+    AccA acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
     // a wavefront takes the periodic path if any of its rows needs it (uniform branch)
     if (__ballot(!away(C, pi)) == 0ull)
-        sweep_a_pairs<SURF, K, false>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w], acc, npair);
+        sweep_a_pairs<SURF, K, false>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w], acc);
     else
-        sweep_a_pairs<SURF, K, true>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w], acc, npair);
-    for (int q = 0; q < 6; ++q)
-        if (SURF || (q != 1 && q != 2 && q != 3)) acc[q] = slot_sum<K>(acc[q]);
-    npair = slot_sum_i<K>(npair);
+        sweep_a_pairs<SURF, K, true>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w], acc);
+    acc.na = slot_sum<K>(acc.na);
+    acc.ws = slot_sum<K>(acc.ws);
+    acc.ds = slot_sum<K>(acc.ds);
+    if (SURF) {
+        acc.gx = slot_sum<K>(acc.gx);
+        acc.gy = slot_sum<K>(acc.gy);
+        acc.gz = slot_sum<K>(acc.gz);
+    }
+    int npair = slot_sum_i<K>(acc.np);
     if (f.slot == 0 && f.valid) {
         const int i = f.i;
         if (count_out[i] == 0)                 // no list (main.cpp:1416): all sums are zero
-            for (int q = 0; q < 6; ++q) acc[q] = 0.0;
-        const double na = acc[0] * C.ca;
-        const double ep = C.cp * acc[4] - C.n0p;
-        const double dv = -C.dcp * acc[5];
+            acc = AccA{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
+        const double na = acc.na * C.ca;
+        const double ep = C.cp * acc.ws - C.n0p;
+        const double dv = -C.dcp * acc.ds;
         const double kap = ep < 0.0 ? 0.0 : C.kbulk[ti];
         const double p = -C.lbulk[ti] * dv + kap * ep;
         double pa = C.cofa[ti] * (na - C.n0a) / C.l0;
         if (C.n0a <= na) pa = 0.0;
         const double gs = C.cg * C.g_scale;
         vel[i].w = p;
-        aux[i] = make_double4(acc[1] * gs, acc[2] * gs, acc[3] * gs, pa);
+        aux[i] = make_double4(acc.gx * gs, acc.gy * gs, acc.gz * gs, pa);
         dens_a[i] = na;
         vstrain[i] = ep;
         divp[i] = dv;
@@ -748,7 +776,10 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
         npair = 0;
     }
     for (int off = 32; off > 0; off >>= 1) npair += __shfl_xor(npair, off);
-    if ((threadIdx.x & 63) == 0 && npair) atomicAdd(pair_count, (unsigned long long)npair);
+    if ((threadIdx.x & 63) == 0 && npair) {
+        const int slot = (int)((blockIdx.x * 4u + (threadIdx.x >> 6)) & (kPairSlots - 1));
+        atomicAdd(pair_count + slot * kPairStride, (unsigned long long)npair);
+    }
 }
 
 // -------------------------------------------------------------------------------------
@@ -761,59 +792,64 @@ struct PairB {
     double4 p, v, a;
 };
 
+struct AccB {
+    double px, py, pz;      // pressure
+    double vx, vy, vz;      // viscosity
+    double ax, ay, az;      // attractive pressure
+    double sx, sy, sz;      // diffuse interface
+};
+
 template <int SURF, int K, bool WRAP>
 __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, const double4& vi, const double4& ai,
                                               const double4* __restrict__ pos, const double4* __restrict__ vel,
-                                              const double4* __restrict__ aux, double4* lds, int* ldsj, double* acc)
+                                              const double4* __restrict__ aux, v4d* lds, int* ldsj, AccB& acc)
 {
     const double mu_i = tab[ti * 4 + ti];
-    auto fetch = [&](int j, double4* r) {
-        r[0] = pos[j];
-        r[1] = vel[j];
-        if (SURF) r[2] = aux[j];
+    auto fetch = [&](int j, Rec& r) {
+        r.p = ld4(pos + j);
+        r.v = ld4(vel + j);
+        if (SURF) r.a = ld4(aux + j);
     };
-    auto eval = [&](const double4* q) {
-        const double4 p = q[0], v = q[1];
-        const double dx = WRAP ? rel_wrap(p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(p.x, pi.x, C.hwid[0]);
-        const double dy = WRAP ? rel_wrap(p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(p.y, pi.y, C.hwid[1]);
-        const double dz = WRAP ? rel_wrap(p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(p.z, pi.z, C.hwid[2]);
+    auto eval = [&](const Rec& q) {
+        const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(q.p.x, pi.x, C.hwid[0]);
+        const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(q.p.y, pi.y, C.hwid[1]);
+        const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(q.p.z, pi.z, C.hwid[2]);
         const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
         const double ir = rsqrt_nr(q2);
         const double r = q2 * ir;
-        const int tj = tag_of(p.w) & 3;
+        const int tj = tag_of(q.p.w) & 3;
         // pressure
-        const double cp = (vi.w + v.w) * fmax(fma(-r, C.inv_hp, 1.0), 0.0) * ir;
-        acc[0] = fma(cp, dx, acc[0]);
-        acc[1] = fma(cp, dy, acc[1]);
-        acc[2] = fma(cp, dz, acc[2]);
+        const double cp = (vi.w + q.v.w) * fmax(fma(-r, C.inv_hp, 1.0), 0.0) * ir;
+        acc.px = fma(cp, dx, acc.px);
+        acc.py = fma(cp, dy, acc.py);
+        acc.pz = fma(cp, dz, acc.pz);
         // viscosity with the harmonic-mean coefficient of the type pair
         const double mu = C.mu_uniform ? mu_i : tab[ti * 4 + tj];
-        const double ud = fma(v.z - vi.z, dz, fma(v.y - vi.y, dy, (v.x - vi.x) * dx));
+        const double ud = fma(q.v.z - vi.z, dz, fma(q.v.y - vi.y, dy, (q.v.x - vi.x) * dx));
         const double cv = mu * ud * fmax(fma(-r, C.inv_hv, 1.0), 0.0) * (ir * ir * ir);
-        acc[3] = fma(cv, dx, acc[3]);
-        acc[4] = fma(cv, dy, acc[4]);
-        acc[5] = fma(cv, dz, acc[5]);
+        acc.vx = fma(cv, dx, acc.vx);
+        acc.vy = fma(cv, dy, acc.vy);
+        acc.vz = fma(cv, dz, acc.vz);
         if (SURF) {
-            const double4 a = q[SURF ? 2 : 0];
             const double rij = C.ratio_one ? 1.0 : tab[16 + ti * 4 + tj];
             const double rji = C.ratio_one ? 1.0 : tab[16 + tj * 4 + ti];
             const double s = r * C.inv_ha;
             const double om = fmax(1.0 - s, 0.0);
-            const double ca = fma(a.w, rji, ai.w * rij) * om * fma(-3.0, s, 1.0) * ir;
-            acc[6] = fma(ca, dx, acc[6]);
-            acc[7] = fma(ca, dy, acc[7]);
-            acc[8] = fma(ca, dz, acc[8]);
+            const double ca = fma(q.a.w, rji, ai.w * rij) * om * fma(-3.0, s, 1.0) * ir;
+            acc.ax = fma(ca, dx, acc.ax);
+            acc.ay = fma(ca, dy, acc.ay);
+            acc.az = fma(ca, dz, acc.az);
             // diffuse interface with Q = R_ji G_j - R_ij G_i; a_j uses the type of i
             // (main.cpp:1792), so a_i factors out of the sum
-            const double qx = fma(rji, a.x, -rij * ai.x);
-            const double qy = fma(rji, a.y, -rij * ai.y);
-            const double qz = fma(rji, a.z, -rij * ai.z);
+            const double qx = fma(rji, q.a.x, -rij * ai.x);
+            const double qy = fma(rji, q.a.y, -rij * ai.y);
+            const double qz = fma(rji, q.a.z, -rij * ai.z);
             const double qd = fma(qz, dz, fma(qy, dy, qx * dx)) * (C.dcg * ir);
             const double w = C.cg * om;
-            acc[9] = fma(-om, fma(qd, dx, w * qx), acc[9]);
-            acc[10] = fma(-om, fma(qd, dy, w * qy), acc[10]);
-            acc[11] = fma(-om, fma(qd, dz, w * qz), acc[11]);
+            acc.sx = fma(-om, fma(qd, dx, w * qx), acc.sx);
+            acc.sy = fma(-om, fma(qd, dy, w * qy), acc.sy);
+            acc.sz = fma(-om, fma(qd, dz, w * qz), acc.sz);
         }
     };
     walk_union_lds<K, SURF ? 3 : 2>(f, lds, ldsj, fetch, eval);
@@ -843,8 +879,8 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     __shared__ double tab[32];
     constexpr int S = Frame<K>::S;
     constexpr int NREC = SURF ? 3 : 2;
-    __shared__ double4 lds[4][2 * NREC * S];  // per wavefront: 2 buffers x records x S
-    __shared__ int ldsj[4][2 * S];
+    __shared__ v4d lds[4][NREC * 64];     // per wavefront: records x 64 entries
+    __shared__ int ldsj[4][64];
     if (threadIdx.x < 16) {
         tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
         tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
@@ -857,28 +893,31 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     const double4 vi = vel[f.i];
     const int ti = tag_of(pi.w) & 3;
     const double4 ai = SURF ? aux[f.i] : make_double4(0.0, 0.0, 0.0, 0.0);
-    constexpr int NA = SURF ? 12 : 6;
-    double acc[NA];
-    for (int q = 0; q < NA; ++q) acc[q] = 0.0;
+    AccB acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (__ballot(!away(C, pi)) == 0ull)
         sweep_b_pairs<SURF, K, false>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], ldsj[w], acc);
     else
         sweep_b_pairs<SURF, K, true>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], ldsj[w], acc);
-    for (int q = 0; q < NA; ++q) acc[q] = slot_sum<K>(acc[q]);
+    acc.px = slot_sum<K>(acc.px); acc.py = slot_sum<K>(acc.py); acc.pz = slot_sum<K>(acc.pz);
+    acc.vx = slot_sum<K>(acc.vx); acc.vy = slot_sum<K>(acc.vy); acc.vz = slot_sum<K>(acc.vz);
+    if (SURF) {
+        acc.ax = slot_sum<K>(acc.ax); acc.ay = slot_sum<K>(acc.ay); acc.az = slot_sum<K>(acc.az);
+        acc.sx = slot_sum<K>(acc.sx); acc.sy = slot_sum<K>(acc.sy); acc.sz = slot_sum<K>(acc.sz);
+    }
     if (f.slot != 0 || !f.valid) return;
     const int i = f.i;
     if (count_out[i] == 0)                     // no list (main.cpp:1416): all sums are zero
-        for (int q = 0; q < NA; ++q) acc[q] = 0.0;
+        acc = AccB{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const double cp = C.dcp * C.vol;                      // pressure
     const double cv = C.visc_c * (-C.dcv) * C.vol;        // viscosity
-    double fx = acc[0] * cp, fy = acc[1] * cp, fz = acc[2] * cp;
+    double fx = acc.px * cp, fy = acc.py * cp, fz = acc.pz * cp;
     if (SURF) {
         const double ca = C.dca * C.vol;
         const double cs = C.adiff[ti] * C.s_scale;
-        fx += acc[6] * ca; fy += acc[7] * ca; fz += acc[8] * ca;
-        fx += acc[9] * cs; fy += acc[10] * cs; fz += acc[11] * cs;
+        fx += acc.ax * ca; fy += acc.ay * ca; fz += acc.az * ca;
+        fx += acc.sx * cs; fy += acc.sy * cs; fz += acc.sz * cs;
     }
-    fx += acc[3] * cv; fy += acc[4] * cv; fz += acc[5] * cv;
+    fx += acc.vx * cv; fy += acc.vy * cv; fz += acc.vz * cv;
     double4 vn = make_double4(vi.x, vi.y, vi.z, 0.0);
     if (ti < 2) {
         const double ms = C.mass[ti];
