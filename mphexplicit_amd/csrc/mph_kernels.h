@@ -575,6 +575,7 @@ __device__ __forceinline__ void walk_union(const Frame<K>& f, Load load, Eval ev
 // native 4-wide vectors: records built from them stay in registers (the HIP double4
 // wrapper type, copied as a whole, can keep a record in private memory)
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ v4d ld4(const double4* p) { return *reinterpret_cast<const v4d*>(p); }
 
@@ -590,15 +591,19 @@ struct Rec {
 // fetched from global memory once per cluster instead of once per member, and the next
 // block's loads are in flight while the current block is evaluated.
 template <int K, int NREC, class Fetch, class Eval>
-__device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v4d* __restrict__ lds, int* __restrict__ ldsj,
+__device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restrict__ lds, int* __restrict__ ldsj,
                                                Fetch fetch, Eval eval)
 {
+    // LDS slice: 2*NREC planes of 64 16-byte halves (x,y | z,w of each record), so the
+    // 16 or 32 distinct entries one read touches are contiguous and free of bank conflicts
     constexpr int S = Frame<K>::S;
     const int lane = (int)(threadIdx.x & 63);
     const int blocks = (f.len + 63) >> 6;
     auto idx = [&](int blk) { const int e = (blk << 6) + lane; return e < f.len ? f.list[e] : -1; };
     Rec r = {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}};
-    int jr = idx(0);
+    // indices run two blocks ahead of the records, so a block's record loads never wait
+    // on its index load
+    int jr = idx(0), jn = idx(1);
     if (jr >= 0) fetch(jr, r);
     for (int blk = 0; blk < blocks; ++blk) {
         // LDS operations of one wavefront complete in issue order: the reads of the
@@ -606,13 +611,19 @@ __device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v4d* __restric
         // before the reads below; the barriers keep the compiler from moving LDS
         // accesses across the block boundary
         __builtin_amdgcn_wave_barrier();
-        lds[0 * 64 + lane] = r.p;
-        lds[1 * 64 + lane] = r.v;
-        if (NREC > 2) lds[2 * 64 + lane] = r.a;
+        lds[0 * 64 + lane] = r.p.xy;
+        lds[1 * 64 + lane] = r.p.zw;
+        lds[2 * 64 + lane] = r.v.xy;
+        lds[3 * 64 + lane] = r.v.zw;
+        if (NREC > 2) {
+            lds[4 * 64 + lane] = r.a.xy;
+            lds[5 * 64 + lane] = r.a.zw;
+        }
         ldsj[lane] = jr;
         __builtin_amdgcn_wave_barrier();
         if (blk + 1 < blocks) {
-            jr = idx(blk + 1);
+            jr = jn;
+            jn = idx(blk + 2);
             if (jr >= 0) fetch(jr, r);
         }
         const int rest = f.len - (blk << 6);
@@ -621,9 +632,14 @@ __device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v4d* __restric
             const int e = t * S + f.slot;
             const int j = ldsj[e];
             Rec q;
-            q.p = lds[0 * 64 + e];
-            q.v = lds[1 * 64 + e];
-            if (NREC > 2) q.a = lds[2 * 64 + e];
+            q.p.xy = lds[0 * 64 + e];
+            q.p.zw = lds[1 * 64 + e];
+            q.v.xy = lds[2 * 64 + e];
+            q.v.zw = lds[3 * 64 + e];
+            if (NREC > 2) {
+                q.a.xy = lds[4 * 64 + e];
+                q.a.zw = lds[5 * 64 + e];
+            }
             if (j >= 0 && j != f.i && f.valid) eval(q);
         }
     }
@@ -670,7 +686,7 @@ template <int SURF, int K, bool WRAP>
 __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, double vix, double viy, double viz,
                                               const double4* __restrict__ pos, const double4* vel,
-                                              v4d* lds, int* ldsj, AccA& acc)
+                                              v2d* lds, int* ldsj, AccA& acc)
 {
     auto fetch = [&](int j, Rec& r) {
         r.p = ld4(pos + j);
@@ -727,7 +743,7 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     const CaseConst& C = cases[m.x];
     __shared__ double tab[32];                // [0,16): mu_ij, [16,32): InteractionRatio
     constexpr int S = Frame<K>::S;
-    __shared__ v4d lds[4][2 * 64];        // per wavefront: {pos, vel} x 64 entries
+    __shared__ v2d lds[4][4 * 64];        // per wavefront: {pos, vel} halves x 64 entries
     __shared__ int ldsj[4][64];
     if (threadIdx.x < 16) {
         tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
@@ -803,7 +819,7 @@ template <int SURF, int K, bool WRAP>
 __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, const double4& vi, const double4& ai,
                                               const double4* __restrict__ pos, const double4* __restrict__ vel,
-                                              const double4* __restrict__ aux, v4d* lds, int* ldsj, AccB& acc)
+                                              const double4* __restrict__ aux, v2d* lds, int* ldsj, AccB& acc)
 {
     const double mu_i = tab[ti * 4 + ti];
     auto fetch = [&](int j, Rec& r) {
@@ -879,7 +895,7 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     __shared__ double tab[32];
     constexpr int S = Frame<K>::S;
     constexpr int NREC = SURF ? 3 : 2;
-    __shared__ v4d lds[4][NREC * 64];     // per wavefront: records x 64 entries
+    __shared__ v2d lds[4][2 * NREC * 64]; // per wavefront: record halves x 64 entries
     __shared__ int ldsj[4][64];
     if (threadIdx.x < 16) {
         tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
