@@ -24,7 +24,9 @@ sys.path.insert(0, ROOT)
 # add/sub/mul/div/sqrt as 1 and fma as 2, min-image free: sweep A / sweep B
 FLOPS_PER_PAIR = {False: (27, 49), True: (43, 117)}   # keyed by "surface tension present"
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+ERROR_FIELDS = ("position", "velocity", "force", "density_a", "vol_strain_p", "pressure_p")
 
 
 def log(*a):
@@ -62,6 +64,60 @@ def cpu_baseline(deck, steps: int, cores: int):
                       % (steps + 1, deck.n, cores, rb)}
 
 
+def timed_run(s, args):
+    """warmup, timed steps (wall clock around the stream), then a HIP-event window"""
+    stepper = s.step_graph if args.graph else s.step
+    stepper(args.warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    stepper(args.steps)
+    s.sync()
+    elapsed = time.perf_counter() - t0
+    k2 = max(10, args.steps // 4)
+    tm = s.step_timed(k2)
+    return args.steps / elapsed, tm, k2, args.warmup + args.steps + k2
+
+
+def field_errors(a, b):
+    """per field: normwise max|a-b|/max|b| and relative L2 |a-b|/|b|"""
+    import numpy as np
+    out = {}
+    for k in ERROR_FIELDS:
+        x, y = a[k].astype(np.float64), b[k].astype(np.float64)
+        d = x - y
+        out[k] = {"normwise": float(np.max(np.abs(d)) / max(np.max(np.abs(y)), 1e-300)),
+                  "rel_l2": float(np.linalg.norm(d) / max(np.linalg.norm(y), 1e-300))}
+    return out
+
+
+def batch_leg(args, Solver):
+    """64 independent uneven cases (2D/3D droplets and bridges, 2e3..6e4 particles each) in
+    one context: every kernel launch covers all cases, steps are replayed HIP graphs"""
+    from mphexplicit_amd import decks
+    cases = decks.batch_cases(count=64)
+    n = [d.n for d in cases]
+    t0 = time.perf_counter()
+    b = Solver.from_decks(cases)
+    create = time.perf_counter() - t0
+    b.step_graph(args.warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step_graph(args.steps)
+    b.sync()
+    el = time.perf_counter() - t0
+    k2 = max(10, args.steps // 4)
+    tm = b.step_timed(k2)
+    st = b.stats()
+    b.close()
+    sps = args.steps / el
+    return {"cases": len(cases), "particles": int(sum(n)), "min_case": int(min(n)), "max_case": int(max(n)),
+            "value": sps, "unit": "steps/s (all cases advance one step)", "ms_per_step": 1e3 / sps,
+            "particle_steps_per_s": sps * sum(n), "pair_interactions_per_s": tm["pairs"] / k2 * sps,
+            "create_s": create, "cluster_size": st["cluster_size"], "graph": True,
+            "phase_ms": {"move": tm["move_ms"] / k2, "rebuild": tm["rebuild_ms"] / k2,
+                         "sweep_a": tm["sweep_a_ms"] / k2, "sweep_b": tm["sweep_b_ms"] / k2}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +128,10 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay a captured HIP graph per step")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-case batch leg")
+    ap.add_argument("--error-steps", type=int, default=500,
+                    help="steps after which the mixed leg's fields are compared with FP64")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -86,17 +146,7 @@ def main():
     t0 = time.perf_counter()
     s = Solver.from_deck(deck, prec)
     log("create %.2fs" % (time.perf_counter() - t0))
-    stepper = s.step_graph if args.graph else s.step
-    stepper(args.warmup)
-    s.sync()
-    t0 = time.perf_counter()
-    stepper(args.steps)
-    s.sync()
-    elapsed = time.perf_counter() - t0
-    steps_per_s = args.steps / elapsed
-    # per-phase device time with HIP events on the solver's stream (separate window)
-    k2 = max(10, args.steps // 4)
-    tm = s.step_timed(k2)
+    steps_per_s, tm, k2, done = timed_run(s, args)
     st = s.stats()
     surface = bool(any(t > 0 for t in deck.data.surface_tension))
     fa, fb = FLOPS_PER_PAIR[surface]
@@ -111,6 +161,27 @@ def main():
                 "note": "FP64 VALU kernel; MI355X's fp64 vector and matrix peaks are both 78.6 TFLOP/s; "
                         "achieved = %d algorithmic flops/pair x %.4g pairs per launch / mean launch time"
                         % (fa if dom == "sweep_a" else fb, pairs_per_step)}
+    mixed = None
+    if prec == MPH_FP64 and not args.no_mixed:
+        # the same deck with FP32 pair math: timing, and the error of every field against the
+        # FP64 run after the same number of steps
+        m = Solver.from_deck(deck, MPH_MIXED)
+        m_sps, m_tm, _, m_done = timed_run(m, args)
+        n_err = max(args.error_steps, done)
+        s.step(n_err - done)
+        m.step(n_err - m_done)
+        errs = field_errors(m.fields(names=ERROR_FIELDS), s.fields(names=ERROR_FIELDS))
+        m_ph = {"sweep_a": m_tm["sweep_a_ms"] / k2, "sweep_b": m_tm["sweep_b_ms"] / k2}
+        m_dom = max(m_ph, key=lambda k: m_ph[k])
+        m_ach = (fa if m_dom == "sweep_a" else fb) * (m_tm["pairs"] / k2) / (m_ph[m_dom] * 1e-3) / 1e12
+        mixed = {"value": m_sps, "unit": "steps/s", "ms_per_step": 1e3 / m_sps,
+                 "speedup_vs_fp64": m_sps / steps_per_s,
+                 "phase_ms": {"move": m_tm["move_ms"] / k2, "rebuild": m_tm["rebuild_ms"] / k2, **m_ph},
+                 "roofline": {"bound": "valu", "achieved": m_ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": m_ach / FP32_PEAK_TFLOPS, "kernel": "k_" + m_dom + "_mixed"},
+                 "error_vs_fp64_after_steps": n_err, "error_vs_fp64": errs}
+        m.close()
+    batch = None if args.no_batch else batch_leg(args, Solver)
     cpu = None
     if not args.no_cpu_baseline:
         cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
@@ -142,6 +213,8 @@ def main():
         "phase_ms": phases,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed": mixed,
+        "batch64": batch,
     }
     if cpu:
         out["speedup_vs_cpu"] = steps_per_s / cpu["value"]

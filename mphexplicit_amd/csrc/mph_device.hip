@@ -21,6 +21,7 @@
 #include "mph_case.h"
 #include "mph_error.h"
 #include "mph_kernels.h"
+#include "mph_mixed.h"
 
 using namespace mph;
 
@@ -78,6 +79,7 @@ struct mph_ctx {
     hipStream_t stream = nullptr;
     int ncases = 0, n = 0, ncells = 0;
     int nkeys = 0, sort_passes = 0, sort_blocks = 0, any_surface = 0;
+    int mixed = 0;                 // MPH_MIXED: FP32 pair sweeps (all cases of a context)
     int K = 4;                 // cluster size (rows per wavefront)
     int width = 0;             // reserved entries per cluster list
     int nslots = 0;            // cluster slots over all cases (capacity)
@@ -398,6 +400,13 @@ static int launch_rebuild(mph_ctx* c, int count_only)
 // that repeated launches leave the state unchanged
 static int launch_sweep_a(mph_ctx* c, int real)
 {
+    if (c->mixed) {
+        LAUNCH_SK(k_sweep_a_mixed, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream,
+                  c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width,
+                  c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p, c->wallc.p,
+                  real ? c->ncases : 0, real ? c->pairs.p : c->pairs_scratch.p);
+        return launch_ok("sweep A");
+    }
     LAUNCH_SK(k_sweep_a, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p,
               c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p,
               c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p, c->wallc.p,
@@ -407,6 +416,13 @@ static int launch_sweep_a(mph_ctx* c, int real)
 
 static int launch_sweep_b(mph_ctx* c, int real)
 {
+    if (c->mixed) {
+        LAUNCH_SK(k_sweep_b_mixed, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream,
+                  c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width,
+                  c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->vnext_out(real), c->force_out(real), c->flags.p,
+                  c->any_flag.p, c->rebuilds.p, real ? c->ncases : 0);
+        return launch_ok("sweep B");
+    }
     LAUNCH_SK(k_sweep_b, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p,
               c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p,
               c->pos.p, c->vel.p, c->aux.p, c->vnext_out(real), c->force_out(real), c->flags.p,
@@ -513,8 +529,8 @@ static std::string tune_key(mph_ctx* c)
     int nb = 0;
     while ((1ll << (nb + 1)) <= (long long)c->n) ++nb;
     char buf[256];
-    std::snprintf(buf, sizeof(buf), "%s|dim%d|surf%d|cases%d|n2^%d", arch.c_str(), dim, c->any_surface,
-                  c->ncases, nb);
+    std::snprintf(buf, sizeof(buf), "%s|dim%d|surf%d|mixed%d|cases%d|n2^%d", arch.c_str(), dim, c->any_surface,
+                  c->mixed, c->ncases, nb);
     std::string k = buf;
     for (auto& ch : k) if (ch == ' ') ch = '_';
     return k;
@@ -622,8 +638,13 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     std::vector<int> hrank, hcell, hcell_case;
     std::vector<int2> logical;
     c->any_surface = 0;
+    c->mixed = prm[0].precision == MPH_MIXED;
     for (int k = 0; k < ncases; ++k) {
         const int nk = counts[k];
+        if (prm[k].precision != MPH_FP64 && prm[k].precision != MPH_MIXED)
+            return fail(MPH_ERR_ARG, "case %d: unknown precision %d", k, prm[k].precision);
+        if ((prm[k].precision == MPH_MIXED) != (c->mixed != 0))
+            return fail(MPH_ERR_ARG, "case %d: all cases of a batch must use the same precision", k);
         int nfluid = 0;
         for (int i = 0; i < nk; ++i) {
             const int t = type[begin + i];

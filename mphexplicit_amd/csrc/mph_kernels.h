@@ -34,12 +34,12 @@ constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
-constexpr int kXcds = 8;
-constexpr int kSubBits = 6;
+constexpr int kXcds = 8;        // MI355X: 8 XCDs x 32 CUs, workgroups dispatched round-robin
+constexpr int kSubBits = 6;     // sub-cell bits of a sort key
 // pair counters: one atomic per wavefront would serialise on a single address, so
 // wavefronts add into kPairSlots counters 128 bytes apart, summed on the host
 constexpr int kPairSlots = 1024;
-constexpr int kPairStride = 16;  // sub-cell bits of a sort key   // MI355X: 8 XCDs x 32 CUs, workgroups dispatched round-robin
+constexpr int kPairStride = 16;
 
 __device__ __forceinline__ int tag_of(double w) { return (int)__double_as_longlong(w); }
 
@@ -528,6 +528,7 @@ struct Frame {
     int len;        // union length
     int row, slot;
     int i;          // member particle of this lane's row
+    int first;      // first member (reference point of the mixed-precision sweeps)
     bool valid;     // row < cluster size
     const int* list;
 };
@@ -547,6 +548,7 @@ __device__ __forceinline__ bool cluster_frame(const CaseConst& C, int first_slot
     f.slot = lane % Frame<K>::S;
     f.valid = f.row < info.y;
     f.i = info.x + (f.valid ? f.row : 0);
+    f.first = info.x;
     f.len = cl_len[f.g];
     f.list = cl_nbr + (size_t)f.g * width;
     return true;

@@ -1,0 +1,387 @@
+// Mixed-precision pair sweeps (MPH_MIXED): pair geometry, kernel weights and force sums
+// in FP32; positions, velocity integration and the density/pressure state in FP64.
+//
+// The weakly compressible pressure P = -lambda D + kappa e^P with e^P = sum w_P - N0p is a
+// difference of two numbers near N0p, so sweep A accumulates its density sums (n^A,
+// sum w_P, the divergence sum) in FP64 from FP32 pair weights; sweep B's force sums stay
+// FP32 and are combined with their FP64 prefactors in the epilogue.  The rest of the step
+// (move, wrap, rebuild, lists, update) is the FP64 path.
+//
+// Geometry: the loader lanes convert each union entry to an FP32 offset from the
+// cluster's first member x_f, r_j = (float)(x_j - x_f); each row holds o_i = (float)(x_i
+// - x_f), and x_ij = r_j - o_i.  Offsets are a few cut-off radii, so the FP32 rounding is
+// about 1e-7 of the interaction radius.  On the periodic path both offsets are minimum
+// images relative to x_f and x_ij is folded once more into [-W/2, W/2) in FP32.  Records
+// are 16-byte float4, half the LDS and register footprint of the FP64 sweep.
+#pragma once
+
+#include "mph_kernels.h"
+
+namespace mph {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct RecF {
+    v4f p, v, a;            // offset+tag bits, velocity+P, gravity centre+PA (a with surface tension)
+};
+
+__device__ __forceinline__ float fold_f(float d, float hw, float w)
+{
+    return d >= hw ? d - w : (d < -hw ? d + w : d);
+}
+
+// FP32 offset of x from the reference point xf on one axis
+template <bool WRAP>
+__device__ __forceinline__ float offset_f(double x, double xf, double hw, double w)
+{
+    return (float)(WRAP ? rel_wrap(x, xf, hw, w) : x - xf);
+}
+
+template <int K, int NREC, class Fetch, class Eval>
+__device__ __forceinline__ void walk_union_lds_f(const Frame<K>& f, v4f* __restrict__ lds, int* __restrict__ ldsj,
+                                                 Fetch fetch, Eval eval)
+{
+    // the FP32 counterpart of walk_union_lds: NREC planes of 64 float4 records
+    constexpr int S = Frame<K>::S;
+    const int lane = (int)(threadIdx.x & 63);
+    const int blocks = (f.len + 63) >> 6;
+    auto idx = [&](int blk) { const int e = (blk << 6) + lane; return e < f.len ? f.list[e] : -1; };
+    RecF r = {v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
+    int jr = idx(0), jn = idx(1);
+    if (jr >= 0) fetch(jr, r);
+    for (int blk = 0; blk < blocks; ++blk) {
+        // in-order LDS operations of one wavefront; the barriers only stop the compiler
+        // from moving LDS accesses across the block boundary
+        __builtin_amdgcn_wave_barrier();
+        lds[0 * 64 + lane] = r.p;
+        lds[1 * 64 + lane] = r.v;
+        if (NREC > 2) lds[2 * 64 + lane] = r.a;
+        ldsj[lane] = jr;
+        __builtin_amdgcn_wave_barrier();
+        if (blk + 1 < blocks) {
+            jr = jn;
+            jn = idx(blk + 2);
+            if (jr >= 0) fetch(jr, r);
+        }
+        const int rest = f.len - (blk << 6);
+        const int nsteps = rest >= 64 ? K : (rest + S - 1) / S;
+        for (int t = 0; t < nsteps; ++t) {
+            const int e = t * S + f.slot;
+            const int j = ldsj[e];
+            RecF q;
+            q.p = lds[0 * 64 + e];
+            q.v = lds[1 * 64 + e];
+            if (NREC > 2) q.a = lds[2 * 64 + e];
+            if (j >= 0 && j != f.i && f.valid) eval(q);
+        }
+    }
+}
+
+// -------------------------------------------------------------------------------------
+// sweep A, mixed: same sums and epilogue as k_sweep_a (main.cpp:1665-1727, 1831-1906,
+// 1729-1740)
+
+template <int SURF, int K, bool WRAP>
+__device__ __forceinline__ void sweep_a_pairs_f(const CaseConst& C, const float* tabf, const Frame<K>& f, int ti,
+                                                const double4& pf, float ox, float oy, float oz,
+                                                float vix, float viy, float viz,
+                                                const double4* __restrict__ pos, const double4* vel,
+                                                v4f* lds, int* ldsj, AccA& acc, float& gxf, float& gyf,
+                                                float& gzf)
+{
+    const float hwx = (float)C.hwid[0], hwy = (float)C.hwid[1], hwz = (float)C.hwid[2];
+    const float wx = (float)C.dwid[0], wy = (float)C.dwid[1], wz = (float)C.dwid[2];
+    const float inv_ha = (float)C.inv_ha, inv_hp = (float)C.inv_hp, hmax2 = (float)C.hmax2;
+    auto fetch = [&](int j, RecF& r) {
+        const double4 p = pos[j];
+        r.p = v4f{offset_f<WRAP>(p.x, pf.x, C.hwid[0], C.dwid[0]), offset_f<WRAP>(p.y, pf.y, C.hwid[1], C.dwid[1]),
+                  offset_f<WRAP>(p.z, pf.z, C.hwid[2], C.dwid[2]), __int_as_float(tag_of(p.w) & 3)};
+        // P (vel.w) of partners is being written by this kernel: read x, y, z only
+        r.v = v4f{(float)vel[j].x, (float)vel[j].y, (float)vel[j].z, 0.f};
+    };
+    auto eval = [&](const RecF& q) {
+        float dx = q.p.x - ox, dy = q.p.y - oy, dz = q.p.z - oz;
+        if (WRAP) {
+            dx = fold_f(dx, hwx, wx);
+            dy = fold_f(dy, hwy, wy);
+            dz = fold_f(dz, hwz, wz);
+        }
+        const float q2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        acc.np += q2 <= hmax2;
+        const float ir = __builtin_amdgcn_rsqf(q2);
+        const float r = q2 * ir;
+        const float sa = r * inv_ha;
+        const float oma = fmaxf(1.f - sa, 0.f);
+        const float ratio = C.ratio_one ? 1.f : tabf[16 + ti * 4 + __float_as_int(q.p.w)];
+        const float wa = ratio * (oma * oma);
+        acc.na += (double)(wa * sa);
+        if (SURF) {
+            gxf = fmaf(dx, wa, gxf);
+            gyf = fmaf(dy, wa, gyf);
+            gzf = fmaf(dz, wa, gzf);
+        }
+        const float omp = fmaxf(fmaf(-r, inv_hp, 1.f), 0.f);
+        acc.ws += (double)(omp * omp);
+        const float ud = fmaf(q.v.z - viz, dz, fmaf(q.v.y - viy, dy, (q.v.x - vix) * dx));
+        acc.ds += (double)(ud * (omp * ir));
+    };
+    walk_union_lds_f<K, 2>(f, lds, ldsj, fetch, eval);
+}
+
+template <int SURF, int K>
+__global__ void __launch_bounds__(256)
+k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
+                const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
+                const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
+                const int* __restrict__ count_out,
+                const double4* __restrict__ pos, double4* vel, double4* __restrict__ aux,
+                double* __restrict__ dens_a, double* __restrict__ vstrain, double* __restrict__ divp,
+                double* __restrict__ wallc, int ncases, unsigned long long* __restrict__ pair_count)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // wall centres advance once per step (main.cpp:2048-2052)
+        for (int c = 0; c < ncases; ++c)
+            for (int t = 2; t < kTypes; ++t)
+                for (int d = 0; d < 3; ++d)
+                    wallc[(c * kTypes + t) * 3 + d] += cases[c].wvel[t][d] * cases[c].dt;
+    }
+    const int2 m = cmap[blockIdx.x];
+    if (m.x < 0) return;
+    const CaseConst& C = cases[m.x];
+    __shared__ float tabf[32];                 // [16,32): InteractionRatio
+    __shared__ v4f lds[4][2 * 64];             // per wavefront: {offset, vel} x 64 entries
+    __shared__ int ldsj[4][64];
+    if (threadIdx.x < 16) {
+        tabf[threadIdx.x] = (float)C.muij[threadIdx.x >> 2][threadIdx.x & 3];
+        tabf[16 + threadIdx.x] = (float)C.ir[threadIdx.x >> 2][threadIdx.x & 3];
+    }
+    __syncthreads();
+    Frame<K> f;
+    if (!cluster_frame<K>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f)) return;
+    const int w = threadIdx.x >> 6;
+    const double4 pi = pos[f.i];
+    const double4 pf = pos[f.first];
+    const int ti = tag_of(pi.w) & 3;
+    const float vix = (float)vel[f.i].x, viy = (float)vel[f.i].y, viz = (float)vel[f.i].z;
+    AccA acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
+    float gxf = 0.f, gyf = 0.f, gzf = 0.f;
+    if (__ballot(!away(C, pi)) == 0ull) {
+        const float ox = offset_f<false>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
+        const float oy = offset_f<false>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
+        const float oz = offset_f<false>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
+        sweep_a_pairs_f<SURF, K, false>(C, tabf, f, ti, pf, ox, oy, oz, vix, viy, viz, pos, vel, lds[w], ldsj[w],
+                                        acc, gxf, gyf, gzf);
+    } else {
+        const float ox = offset_f<true>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
+        const float oy = offset_f<true>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
+        const float oz = offset_f<true>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
+        sweep_a_pairs_f<SURF, K, true>(C, tabf, f, ti, pf, ox, oy, oz, vix, viy, viz, pos, vel, lds[w], ldsj[w],
+                                       acc, gxf, gyf, gzf);
+    }
+    acc.na = slot_sum<K>(acc.na);
+    acc.ws = slot_sum<K>(acc.ws);
+    acc.ds = slot_sum<K>(acc.ds);
+    if (SURF) {
+        acc.gx = slot_sum<K>((double)gxf);
+        acc.gy = slot_sum<K>((double)gyf);
+        acc.gz = slot_sum<K>((double)gzf);
+    }
+    int npair = slot_sum_i<K>(acc.np);
+    if (f.slot == 0 && f.valid) {
+        const int i = f.i;
+        if (count_out[i] == 0)                 // no list (main.cpp:1416): all sums are zero
+            acc = AccA{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
+        const double na = acc.na * C.ca;
+        const double ep = C.cp * acc.ws - C.n0p;
+        const double dv = -C.dcp * acc.ds;
+        const double kap = ep < 0.0 ? 0.0 : C.kbulk[ti];
+        const double p = -C.lbulk[ti] * dv + kap * ep;
+        double pa = C.cofa[ti] * (na - C.n0a) / C.l0;
+        if (C.n0a <= na) pa = 0.0;
+        const double gs = C.cg * C.g_scale;
+        vel[i].w = p;
+        aux[i] = make_double4(acc.gx * gs, acc.gy * gs, acc.gz * gs, pa);
+        dens_a[i] = na;
+        vstrain[i] = ep;
+        divp[i] = dv;
+    } else {
+        npair = 0;
+    }
+    for (int off = 32; off > 0; off >>= 1) npair += __shfl_xor(npair, off);
+    if ((threadIdx.x & 63) == 0 && npair) {
+        const int slot = (int)((blockIdx.x * 4u + (threadIdx.x >> 6)) & (kPairSlots - 1));
+        atomicAdd(pair_count + slot * kPairStride, (unsigned long long)npair);
+    }
+}
+
+// -------------------------------------------------------------------------------------
+// sweep B, mixed: force terms of k_sweep_b (main.cpp:1908-1938, 1742-1829, 1941-1985),
+// gravity and the FP64 velocity update (main.cpp:1988-2010)
+
+struct AccBF {
+    float px, py, pz, vx, vy, vz, ax, ay, az, sx, sy, sz;
+};
+
+template <int SURF, int K, bool WRAP>
+__device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float* tabf, const Frame<K>& f, int ti,
+                                                const double4& pf, float ox, float oy, float oz,
+                                                const v4f& vi, const v4f& ai,
+                                                const double4* __restrict__ pos, const double4* __restrict__ vel,
+                                                const double4* __restrict__ aux, v4f* lds, int* ldsj, AccBF& acc)
+{
+    const float hwx = (float)C.hwid[0], hwy = (float)C.hwid[1], hwz = (float)C.hwid[2];
+    const float wx = (float)C.dwid[0], wy = (float)C.dwid[1], wz = (float)C.dwid[2];
+    const float inv_ha = (float)C.inv_ha, inv_hp = (float)C.inv_hp, inv_hv = (float)C.inv_hv;
+    const float dcg = (float)C.dcg, cg = (float)C.cg;
+    const float mu_i = tabf[ti * 4 + ti];
+    auto fetch = [&](int j, RecF& r) {
+        const double4 p = pos[j];
+        r.p = v4f{offset_f<WRAP>(p.x, pf.x, C.hwid[0], C.dwid[0]), offset_f<WRAP>(p.y, pf.y, C.hwid[1], C.dwid[1]),
+                  offset_f<WRAP>(p.z, pf.z, C.hwid[2], C.dwid[2]), __int_as_float(tag_of(p.w) & 3)};
+        const double4 u = vel[j];
+        r.v = v4f{(float)u.x, (float)u.y, (float)u.z, (float)u.w};
+        if (SURF) {
+            const double4 a = aux[j];
+            r.a = v4f{(float)a.x, (float)a.y, (float)a.z, (float)a.w};
+        }
+    };
+    auto eval = [&](const RecF& q) {
+        float dx = q.p.x - ox, dy = q.p.y - oy, dz = q.p.z - oz;
+        if (WRAP) {
+            dx = fold_f(dx, hwx, wx);
+            dy = fold_f(dy, hwy, wy);
+            dz = fold_f(dz, hwz, wz);
+        }
+        const float q2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float ir = __builtin_amdgcn_rsqf(q2);
+        const float r = q2 * ir;
+        const int tj = __float_as_int(q.p.w);
+        const float cp = (vi.w + q.v.w) * fmaxf(fmaf(-r, inv_hp, 1.f), 0.f) * ir;
+        acc.px = fmaf(cp, dx, acc.px);
+        acc.py = fmaf(cp, dy, acc.py);
+        acc.pz = fmaf(cp, dz, acc.pz);
+        const float mu = C.mu_uniform ? mu_i : tabf[ti * 4 + tj];
+        const float ud = fmaf(q.v.z - vi.z, dz, fmaf(q.v.y - vi.y, dy, (q.v.x - vi.x) * dx));
+        const float cv = mu * ud * fmaxf(fmaf(-r, inv_hv, 1.f), 0.f) * (ir * ir * ir);
+        acc.vx = fmaf(cv, dx, acc.vx);
+        acc.vy = fmaf(cv, dy, acc.vy);
+        acc.vz = fmaf(cv, dz, acc.vz);
+        if (SURF) {
+            const float rij = C.ratio_one ? 1.f : tabf[16 + ti * 4 + tj];
+            const float rji = C.ratio_one ? 1.f : tabf[16 + tj * 4 + ti];
+            const float s = r * inv_ha;
+            const float om = fmaxf(1.f - s, 0.f);
+            const float ca = fmaf(q.a.w, rji, ai.w * rij) * om * fmaf(-3.f, s, 1.f) * ir;
+            acc.ax = fmaf(ca, dx, acc.ax);
+            acc.ay = fmaf(ca, dy, acc.ay);
+            acc.az = fmaf(ca, dz, acc.az);
+            // Q = R_ji G_j - R_ij G_i; a_j uses the type of i (main.cpp:1792)
+            const float qx = fmaf(rji, q.a.x, -rij * ai.x);
+            const float qy = fmaf(rji, q.a.y, -rij * ai.y);
+            const float qz = fmaf(rji, q.a.z, -rij * ai.z);
+            const float qd = fmaf(qz, dz, fmaf(qy, dy, qx * dx)) * (dcg * ir);
+            const float wg = cg * om;
+            acc.sx = fmaf(-om, fmaf(qd, dx, wg * qx), acc.sx);
+            acc.sy = fmaf(-om, fmaf(qd, dy, wg * qy), acc.sy);
+            acc.sz = fmaf(-om, fmaf(qd, dz, wg * qz), acc.sz);
+        }
+    };
+    walk_union_lds_f<K, SURF ? 3 : 2>(f, lds, ldsj, fetch, eval);
+}
+
+template <int K>
+__device__ __forceinline__ float slot_sum_f(float v)
+{
+    for (int off = Frame<K>::S / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, Frame<K>::S);
+    return v;
+}
+
+template <int SURF, int K>
+__global__ void __launch_bounds__(256)
+k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
+                const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
+                const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
+                const int* __restrict__ count_out,
+                const double4* __restrict__ pos, const double4* __restrict__ vel,
+                const double4* __restrict__ aux, double4* __restrict__ vnext,
+                double4* __restrict__ force, int* __restrict__ flags,
+                int* __restrict__ any_flag, long long* __restrict__ rebuilds, int ncases)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0 && ncases > 0) {
+        for (int c = 0; c < ncases; ++c) {
+            if (flags[c]) rebuilds[c] += 1;
+            flags[c] = 0;
+        }
+        *any_flag = 0;
+    }
+    const int2 m = cmap[blockIdx.x];
+    if (m.x < 0) return;
+    const CaseConst& C = cases[m.x];
+    __shared__ float tabf[32];
+    constexpr int NREC = SURF ? 3 : 2;
+    __shared__ v4f lds[4][NREC * 64];          // per wavefront: records x 64 entries
+    __shared__ int ldsj[4][64];
+    if (threadIdx.x < 16) {
+        tabf[threadIdx.x] = (float)C.muij[threadIdx.x >> 2][threadIdx.x & 3];
+        tabf[16 + threadIdx.x] = (float)C.ir[threadIdx.x >> 2][threadIdx.x & 3];
+    }
+    __syncthreads();
+    Frame<K> f;
+    if (!cluster_frame<K>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f)) return;
+    const int w = threadIdx.x >> 6;
+    const double4 pi = pos[f.i];
+    const double4 pf = pos[f.first];
+    const double4 vi = vel[f.i];
+    const int ti = tag_of(pi.w) & 3;
+    const double4 ai = SURF ? aux[f.i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const v4f vif = v4f{(float)vi.x, (float)vi.y, (float)vi.z, (float)vi.w};
+    const v4f aif = v4f{(float)ai.x, (float)ai.y, (float)ai.z, (float)ai.w};
+    AccBF acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (__ballot(!away(C, pi)) == 0ull) {
+        const float ox = offset_f<false>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
+        const float oy = offset_f<false>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
+        const float oz = offset_f<false>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
+        sweep_b_pairs_f<SURF, K, false>(C, tabf, f, ti, pf, ox, oy, oz, vif, aif, pos, vel, aux, lds[w], ldsj[w],
+                                        acc);
+    } else {
+        const float ox = offset_f<true>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
+        const float oy = offset_f<true>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
+        const float oz = offset_f<true>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
+        sweep_b_pairs_f<SURF, K, true>(C, tabf, f, ti, pf, ox, oy, oz, vif, aif, pos, vel, aux, lds[w], ldsj[w],
+                                       acc);
+    }
+    acc.px = slot_sum_f<K>(acc.px); acc.py = slot_sum_f<K>(acc.py); acc.pz = slot_sum_f<K>(acc.pz);
+    acc.vx = slot_sum_f<K>(acc.vx); acc.vy = slot_sum_f<K>(acc.vy); acc.vz = slot_sum_f<K>(acc.vz);
+    if (SURF) {
+        acc.ax = slot_sum_f<K>(acc.ax); acc.ay = slot_sum_f<K>(acc.ay); acc.az = slot_sum_f<K>(acc.az);
+        acc.sx = slot_sum_f<K>(acc.sx); acc.sy = slot_sum_f<K>(acc.sy); acc.sz = slot_sum_f<K>(acc.sz);
+    }
+    if (f.slot != 0 || !f.valid) return;
+    const int i = f.i;
+    if (count_out[i] == 0)                     // no list (main.cpp:1416): all sums are zero
+        acc = AccBF{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const double cp = C.dcp * C.vol;
+    const double cv = C.visc_c * (-C.dcv) * C.vol;
+    double fx = (double)acc.px * cp, fy = (double)acc.py * cp, fz = (double)acc.pz * cp;
+    if (SURF) {
+        const double ca = C.dca * C.vol;
+        const double cs = C.adiff[ti] * C.s_scale;
+        fx += (double)acc.ax * ca; fy += (double)acc.ay * ca; fz += (double)acc.az * ca;
+        fx += (double)acc.sx * cs; fy += (double)acc.sy * cs; fz += (double)acc.sz * cs;
+    }
+    fx += (double)acc.vx * cv; fy += (double)acc.vy * cv; fz += (double)acc.vz * cv;
+    double4 vn = make_double4(vi.x, vi.y, vi.z, 0.0);
+    if (ti < 2) {
+        const double ms = C.mass[ti];
+        fx += ms * C.grav[0];
+        fy += ms * C.grav[1];
+        fz += ms * C.grav[2];
+        vn.x += fx / ms * C.dt;
+        vn.y += fy / ms * C.dt;
+        vn.z += fz / ms * C.dt;
+    }
+    force[i] = make_double4(fx, fy, fz, 0.0);
+    vnext[i] = vn;
+}
+
+}  // namespace mph

@@ -1,0 +1,92 @@
+"""Mixed precision (MPH_MIXED: FP32 pair geometry and force sums, FP64 state) against the
+FP64 path on the same decks.
+
+Lists and the move are FP64 in both, so at the first step positions and neighbour sets are
+identical; the fields then differ by FP32 rounding of the pair terms.  The bounds below are
+normwise (max |mixed - fp64| / max |fp64|) and hold with a margin of ten or more over the
+errors these decks show (velocity after one step of the dam break, which starts at rest, is
+the largest: 3.7e-5).  Pressure and volumetric strain are differences of sums near N0p
+(P = -lambda D + kappa (sum w_P - N0p), main.cpp:1850-1906) and are compared at the scale of
+a unit strain, K * N0p, like the FP64 parity tests do where the field is at rounding level.
+"""
+import numpy as np
+import pytest
+
+from mphexplicit_amd import decks
+from oracle.oracle import Oracle
+from tests.util import normwise
+
+pytestmark = pytest.mark.gpu
+
+DECKS = {
+    "droplet2d": decks.small_droplet_2d,
+    "movingwall2d": decks.small_moving_wall_2d,
+    "dambreak3d": decks.small_dam_break_3d,
+    "twofluid3d": decks.small_two_fluid_3d,
+}
+
+# per-field bounds after one step and after 20 steps (normwise, see above)
+TOL_STEP1 = {"velocity": 5e-4, "force": 5e-4, "density_a": 1e-6, "vol_strain_p": 1e-5,
+             "pressure_p": 1e-5, "gravity_center": 1e-4}
+TOL_STEP20 = {"position": 1e-6, "velocity": 1e-4, "force": 1e-3, "density_a": 1e-5,
+              "vol_strain_p": 1e-4, "pressure_p": 1e-4}
+
+
+@pytest.fixture(scope="module")
+def solver_mod(gpu_available):
+    if not gpu_available:
+        pytest.skip("no GPU")
+    from mphexplicit_amd import solver
+    return solver
+
+
+def _errors(fm, fd, deck, names):
+    k_n0p = max(deck.data.bulk_modulus) * Oracle(deck).scalars()[1]
+    out = {}
+    for k in names:
+        a, b = fm[k], fd[k]
+        if k in ("vol_strain_p", "pressure_p"):
+            scale = k_n0p if k == "pressure_p" else Oracle(deck).scalars()[1]
+            out[k] = float(np.max(np.abs(a - b)) / max(scale, np.max(np.abs(b))))
+        else:
+            out[k] = normwise(a, b)
+    return out
+
+
+@pytest.mark.parametrize("name", list(DECKS))
+def test_mixed_first_step(solver_mod, name):
+    print("case", name, flush=True)
+    d = DECKS[name]()
+    s64 = solver_mod.Solver.from_deck(d, precision=solver_mod.MPH_FP64)
+    s32 = solver_mod.Solver.from_deck(d, precision=solver_mod.MPH_MIXED)
+    s64.step(1)
+    s32.step(1)
+    f64, f32 = s64.fields(), s32.fields()
+    assert np.array_equal(f64["position"], f32["position"])
+    assert np.array_equal(f64["neighbor_count"], f32["neighbor_count"])
+    names = [k for k in TOL_STEP1 if k != "gravity_center" or any(t != 0.0 for t in d.data.surface_tension)]
+    err = _errors(f32, f64, d, names)
+    print(name, "mixed step 1", " ".join("%s=%.2e" % kv for kv in err.items()), flush=True)
+    bad = {k: e for k, e in err.items() if not e <= TOL_STEP1[k]}
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("name", ["droplet2d", "twofluid3d"])
+def test_mixed_multi_step(solver_mod, name):
+    d = DECKS[name]()
+    s64 = solver_mod.Solver.from_deck(d, precision=solver_mod.MPH_FP64)
+    s32 = solver_mod.Solver.from_deck(d, precision=solver_mod.MPH_MIXED)
+    s64.step(20)
+    s32.step(20)
+    err = _errors(s32.fields(), s64.fields(), d, list(TOL_STEP20))
+    print(name, "mixed step 20", " ".join("%s=%.2e" % kv for kv in err.items()), flush=True)
+    bad = {k: e for k, e in err.items() if not e <= TOL_STEP20[k]}
+    assert not bad, bad
+
+
+def test_mixed_batch_rejects_mixed_precisions(solver_mod):
+    a, b = decks.small_droplet_2d(), decks.small_droplet_2d()
+    from mphexplicit_amd.abi import params_from_deck
+    with pytest.raises(solver_mod.MphError):
+        solver_mod.Solver([params_from_deck(a, 0), params_from_deck(b, 1)], [a.type, b.type], [a.x, b.x],
+                          [a.v, b.v])
