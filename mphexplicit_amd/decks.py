@@ -329,6 +329,12 @@ def small_dam_break_3d(seed=12) -> Deck:
                         name="small_dambreak3d")
 
 
+def drift_dam_break_3d(seed=14) -> Deck:
+    """Small 3D dam break for 1000-step runs: 8x11x8 fluid in a 14x11x12 walled box."""
+    return dam_break_3d(fluid=(8, 11, 8), box=(14, 11, 12), jitter=0.01, seed=seed, steps=1000,
+                        name="drift_dambreak3d")
+
+
 def small_two_fluid_3d(seed=13) -> Deck:
     """3D two-fluid block with walls, surface tension and non-unit interaction ratios."""
     d = dam_break_3d(fluid=(10, 12, 10), box=(16, 12, 14), jitter=0.02, seed=seed, steps=50,

@@ -82,6 +82,25 @@ def test_oracle_bitwise_vs_reference_build(factory, steps):
     assert np.array_equal(o.virial(), r.virial())
 
 
+@pytest.mark.skipif(not reference_available(3), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("factory", [decks.small_droplet_2d, decks.drift_dam_break_3d])
+def test_oracle_1000_steps_bitwise_vs_reference_build(factory):
+    # the whole 1000-step trajectory, rebuild timing included (226 rebuilds for the dam
+    # break), is bit-identical: the oracle's mass/momentum drift is the reference's
+    d = factory()
+    if not reference_available(d.dim):
+        pytest.skip("reference build for %dD not present" % d.dim)
+    o, r = Oracle(d, nthreads=8), Reference(d, nthreads=8)
+    ro = rr = 0
+    for _ in range(4):
+        ro += o.step(250)
+        rr += r.step(250)
+        fo, fr = o.fields(), r.fields()
+        assert np.array_equal(fo["x"], fr["x"]) and np.array_equal(fo["v"], fr["v"])
+    assert ro == rr
+    assert o.time == r.time
+
+
 # ----------------------------------------------------------------- closed forms -----
 
 @pytest.mark.parametrize("dim", [2, 3])
