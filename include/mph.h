@@ -196,8 +196,99 @@ int mph_get_stats(const mph_ctx *ctx, mph_stats *out);
 
 void mph_destroy(mph_ctx *ctx);
 
+/* Change the time step of one case (the reference reads Dt once from the .data file,
+ * main.cpp:502, and keeps it for the run).  Takes effect at the next step; captured
+ * graphs stay valid (they read the case constants from device memory). */
+int mph_set_time_step(mph_ctx *ctx, int case_id, double dt);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *mph_last_error(void);
+
+/* ------------------------------------------------------------------------------------------
+ * Learned part: a pairwise-message surrogate.  No counterpart in the reference.
+ *
+ * For each fluid particle i and each neighbour j within the largest interaction radius h
+ * (the reference's neighbour set, main.cpp:1408), a 17-input MLP (features below; three
+ * hidden layers of 64 SiLU units; 3 outputs) gives a message m_ij, and the predicted mean
+ * acceleration over two steps is a_i = a_scale * sum_j m_ij.  Weights are BF16 on the
+ * matrix cores with FP32 accumulation; master weights and Adam state are FP32.
+ *
+ * Features of a pair (x_ij = periodic x_j - x_i, r = |x_ij|, u = (v_j - v_i)/u_scale):
+ *   0-2 x_ij/r   3 r/h   4-6 u   7 u.(x_ij/r)   8-10 class of i   11-13 class of j
+ *   (one-hot: type 0, type 1, wall)   14 dens_scale e^P_i / N0p, the density deviation that
+ *   drives the pressure (VolStrainP = sum w_P - N0p, main.cpp:1850-1856)   15 the same for j
+ *   16 the constant 1 (carries the first layer's bias)
+ *
+ * A surrogate step advances 2 dt: the plain move to x_{n+1}, sweep A's densities at x_{n+1},
+ * the MLP at (x_{n+1}, v_n), v_{n+1} = v_n + dt (a + g), the plain move to x_{n+2} and
+ * v_{n+2} = v_{n+1} + dt (a + g): two plain steps whose force evaluations are replaced by
+ * one prediction.  Training targets come from the plain trajectory:
+ * a = (v_{n+2} - v_n) / (2 dt) - g at the state (x_{n+1}, v_n).
+ *
+ * Parameter layout (MPH_SUR_NPARAMS floats, row-major, [in][out]):
+ *   W1[17][64] (row 16 = bias of layer 1), W2[64][64], b2[64], W3[64][64], b3[64],
+ *   W4[64][3], b4[3].
+ * ---------------------------------------------------------------------------------------- */
+
+#define MPH_SUR_NPARAMS 9603
+
+typedef struct mph_surrogate mph_surrogate;
+
+typedef struct mph_sur_params {
+    double u_scale;          /* velocity scale of the relative-velocity features [m/s] */
+    double a_scale;          /* acceleration scale of the messages [m/s^2] */
+    double dens_scale;       /* factor on the density deviation e^P / N0p */
+    double lr, beta1, beta2, eps;   /* Adam */
+    int train_stride;        /* training uses fluid particles with sorted index % stride == 0 */
+    int pad0;
+    unsigned long long seed; /* weight initialisation */
+} mph_sur_params;
+
+typedef struct mph_sur_stats {
+    long long blocks;        /* judged blocks */
+    long long rollbacks;     /* blocks replayed with plain steps */
+    long long sur_steps;     /* accepted surrogate steps (2 dt each) */
+    long long plain_steps;   /* plain steps (judges and replays) */
+    double max_err_v;        /* largest relative L2 velocity error seen by a judge */
+    double max_err_n;        /* largest relative L2 number-density error seen by a judge */
+    double last_err_v, last_err_n;
+    double sim_time;         /* simulated time advanced by the call */
+} mph_sur_stats;
+
+void mph_sur_params_default(mph_sur_params *p);
+
+/* Create a surrogate bound to a single-case FP64 context with static walls.  The weights are
+ * initialised from p->seed.  Returns NULL on error (mph_last_error). */
+mph_surrogate *mph_sur_create(mph_ctx *ctx, const mph_sur_params *p);
+
+int mph_sur_get_weights(mph_surrogate *s, float *w);        /* MPH_SUR_NPARAMS floats */
+int mph_sur_set_weights(mph_surrogate *s, const float *w);
+
+/* Acceleration the MLP predicts at the context's current state (positions, velocities and
+ * VolStrainP as mph_get_fields reports them), by original particle id, [n][3]; 0 for walls. */
+int mph_sur_predict(mph_surrogate *s, double *accel);
+
+/* Loss and gradient at the context's current state (as mph_sur_predict) for the given
+ * per-particle targets [n][3] by original id, over the training particles:
+ * loss = mean over those fluid particles of |a - target|^2 / a_scale^2.
+ * grad: MPH_SUR_NPARAMS floats (may be NULL). */
+int mph_sur_gradient(mph_surrogate *s, const double *target, double *loss, float *grad);
+
+/* Train on the context's own plain trajectory: each sample advances the context by two
+ * plain steps and then takes `iters` Adam steps on it.  losses: samples*iters values
+ * (may be NULL). */
+int mph_sur_train(mph_surrogate *s, int samples, int iters, double *losses);
+
+/* Surrogate steps (2 dt each) without a judge. */
+int mph_sur_step(mph_surrogate *s, int nsteps);
+
+/* Judged run: `blocks` blocks of k surrogate steps.  After each block one surrogate step
+ * and two plain steps start from the same state; if the relative L2 difference of the fluid
+ * velocities or number densities exceeds tol, the block is rolled back and replayed with
+ * 2k plain steps; otherwise the plain result is kept.  Accumulates into *st. */
+int mph_sur_advance(mph_surrogate *s, int blocks, int k, double tol, mph_sur_stats *st);
+
+void mph_sur_destroy(mph_surrogate *s);
 
 #ifdef __cplusplus
 }

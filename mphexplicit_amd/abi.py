@@ -79,6 +79,26 @@ class MphTiming(C.Structure):
     ]
 
 
+class MphSurParams(C.Structure):
+    """mph_sur_params of include/mph.h"""
+    _fields_ = [
+        ("u_scale", C.c_double), ("a_scale", C.c_double), ("dens_scale", C.c_double),
+        ("lr", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double),
+        ("train_stride", C.c_int), ("pad0", C.c_int), ("seed", C.c_ulonglong),
+    ]
+
+
+class MphSurStats(C.Structure):
+    """mph_sur_stats of include/mph.h"""
+    _fields_ = [
+        ("blocks", C.c_longlong), ("rollbacks", C.c_longlong), ("sur_steps", C.c_longlong),
+        ("plain_steps", C.c_longlong), ("max_err_v", C.c_double), ("max_err_n", C.c_double),
+        ("last_err_v", C.c_double), ("last_err_n", C.c_double), ("sim_time", C.c_double),
+    ]
+
+
+SUR_NPARAMS = 9603
+
 FIELD_SHAPES = {
     "particle_index": (np.int32, 1), "type": (np.int32, 1), "position": (np.float64, 3),
     "velocity": (np.float64, 3), "force": (np.float64, 3), "density_a": (np.float64, 1),

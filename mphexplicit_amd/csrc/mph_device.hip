@@ -22,6 +22,7 @@
 #include "mph_error.h"
 #include "mph_kernels.h"
 #include "mph_mixed.h"
+#include "mph_surrogate.h"
 
 using namespace mph;
 
@@ -917,6 +918,18 @@ extern "C" int mph_get_fields(mph_ctx* c, int case_id, int order, mph_fields* ou
     return MPH_OK;
 }
 
+extern "C" int mph_set_time_step(mph_ctx* c, int case_id, double dt)
+{
+    if (!c || case_id < 0 || case_id >= c->ncases || !(dt > 0.0) || !std::isfinite(dt))
+        return fail(MPH_ERR_ARG, "mph_set_time_step: bad argument");
+    c->cases[case_id].prm.dt = dt;
+    c->cases[case_id].k.dt = dt;
+    HIP_TRY(hipMemcpyAsync(c->d_cases.p + case_id, &c->cases[case_id].k, sizeof(CaseConst), hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MPH_OK;
+}
+
 extern "C" int mph_compute_virial(mph_ctx* c)
 {
     if (!c) return fail(MPH_ERR_ARG, "null context");
@@ -1018,4 +1031,506 @@ extern "C" void mph_destroy(mph_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     delete c;
+}
+
+// ---------------------------------------------------------------------------------------
+// Pairwise-message surrogate (no counterpart in the reference; see mph_surrogate.h)
+
+namespace {
+
+struct SurCkpt {
+    DevBuf<double4> pos, vel, vnext, snap;
+    DevBuf<int> pidx;
+    DevBuf<double> wallc;
+    std::vector<double> time;
+    long long steps = 0;
+};
+
+}  // namespace
+
+struct mph_surrogate {
+    mph_ctx* c = nullptr;
+    mph_sur_params prm{};
+    long long t_adam = 0;
+    int wh = 0;
+    DevBuf<float> w, m1, m2, grad, wr;
+    DevBuf<bf16x8> pk;
+    DevBuf<int> pcnt, pnbr;
+    DevBuf<float4> apred;                 // by original id (inference) or list entry (training)
+    // training set of the current sample
+    DevBuf<int> plist, toff, towner;
+    int nlist = 0, ntiles = 0;
+    long nslots = 0, cap_slots = 0;
+    DevBuf<__bf16> X;
+    DevBuf<float> Z1, Z2, Z3, dz1, dz2, dz3, dm;
+    DevBuf<float4> target, dmk;
+    DevBuf<double> loss, sums;
+    // snapshot of a training state and original-order scratch
+    DevBuf<double4> spos, svel, v_orig, v_ref;
+    DevBuf<double> sdens, n_ref;
+    DevBuf<int> spidx;
+    SurCkpt ck[2];
+};
+
+extern "C" void mph_sur_params_default(mph_sur_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->u_scale = 0.1;
+    p->a_scale = 10.0;
+    p->dens_scale = 50.0;
+    p->lr = 1e-3;
+    p->beta1 = 0.9;
+    p->beta2 = 0.999;
+    p->eps = 1e-8;
+    p->train_stride = 1;
+    p->seed = 12345;
+}
+
+static SurScales sur_scales(const mph_surrogate* s)
+{
+    const CaseConst& k = s->c->cases[0].k;
+    SurScales sc;
+    sc.inv_u = (float)(1.0 / s->prm.u_scale);
+    sc.a_scale = (float)s->prm.a_scale;
+    sc.dens_scale = (float)s->prm.dens_scale;
+    sc.inv_h = (float)(1.0 / std::sqrt(k.hmax2));
+    sc.inv_n0p = (float)(1.0 / k.n0p);
+    sc.dt = (float)k.dt;
+    sc.gx = (float)k.grav[0];
+    sc.gy = (float)k.grav[1];
+    sc.gz = (float)k.grav[2];
+    return sc;
+}
+
+static int sur_pack(mph_surrogate* s)
+{
+    hipLaunchKernelGGL(k_sur_pack, dim3((kPkTiles * 64 + 255) / 256), dim3(256), 0, s->c->stream, s->w.p, s->pk.p,
+                       s->wr.p);
+    return launch_ok("surrogate pack");
+}
+
+// pairs within the interaction radius for every fluid particle, from the current lists
+static int sur_pairs(mph_surrogate* s)
+{
+    mph_ctx* c = s->c;
+    const long threads = (long)c->nslots * c->K;
+    hipLaunchKernelGGL(k_sur_pairs, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream, c->d_cases.p,
+                       c->slot_case.p, c->nslots, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width,
+                       c->K, c->pos.p, s->wh, s->pcnt.p, s->pnbr.p);
+    return launch_ok("surrogate pairs");
+}
+
+static int sur_grid(const mph_ctx* c)
+{
+    hipDeviceProp_t prop;
+    int cus = 256;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) cus = prop.multiProcessorCount;
+    return cus * 4;     // persistent: four 256-thread blocks per CU
+}
+
+// inference forward at (pos, velf, vstrain) over all fluid particles: s->apred by original id
+static int sur_infer(mph_surrogate* s, const double4* velf)
+{
+    mph_ctx* c = s->c;
+    hipLaunchKernelGGL((k_sur_forward<false>), dim3(sur_grid(c)), dim3(256), 0, c->stream, c->d_cases.p, c->n,
+                       nullptr, 0, nullptr, c->pos.p, velf, c->vstrain.p, c->pidx.p, s->pcnt.p, s->pnbr.p, s->wh,
+                       s->pk.p, s->w.p, sur_scales(s), s->apred.p, nullptr, nullptr, nullptr, nullptr);
+    return launch_ok("surrogate forward");
+}
+
+static int sur_kick(mph_surrogate* s)
+{
+    mph_ctx* c = s->c;
+    hipLaunchKernelGGL(k_sur_kick, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->d_cases.p, c->n, c->pos.p,
+                       c->vel.p, c->pidx.p, s->apred.p, c->vnext.p, c->flags.p, c->any_flag.p, c->rebuilds.p);
+    return launch_ok("surrogate kick");
+}
+
+// one surrogate step (2 dt): move, lists, densities, prediction, kick, move, lists, kick
+static int sur_step(mph_surrogate* s)
+{
+    mph_ctx* c = s->c;
+    int rc = 0;
+    if ((rc = launch_move(c)) || (rc = launch_rebuild(c, 0)) || (rc = launch_sweep_a(c, 1)) || (rc = sur_pairs(s)) ||
+        (rc = sur_infer(s, c->vel.p)) || (rc = sur_kick(s)) || (rc = launch_move(c)) || (rc = launch_rebuild(c, 0)) ||
+        (rc = sur_kick(s)))
+        return rc;
+    for (auto& ch : c->cases) ch.time += 2.0 * ch.prm.dt;
+    c->steps += 2;
+    return MPH_OK;
+}
+
+static int sur_ckpt_save(mph_surrogate* s, SurCkpt& k)
+{
+    mph_ctx* c = s->c;
+    const size_t n = c->n;
+    int rc = 0;
+    if (!k.pos.p && ((rc = k.pos.alloc(n)) || (rc = k.vel.alloc(n)) || (rc = k.vnext.alloc(n)) || (rc = k.snap.alloc(n)) ||
+                     (rc = k.pidx.alloc(n)) || (rc = k.wallc.alloc(c->wallc.n))))
+        return rc;
+    hipStream_t st = c->stream;
+    HIP_TRY(hipMemcpyAsync(k.pos.p, c->pos.p, n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(k.vel.p, c->vel.p, n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(k.vnext.p, c->vnext.p, n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(k.snap.p, c->snap.p, n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(k.pidx.p, c->pidx.p, n * sizeof(int), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(k.wallc.p, c->wallc.p, c->wallc.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    k.time.clear();
+    for (auto& ch : c->cases) k.time.push_back(ch.time);
+    k.steps = c->steps;
+    return MPH_OK;
+}
+
+// restore a checkpoint; the next step rebuilds the cells and lists from the restored state
+static int sur_ckpt_load(mph_surrogate* s, const SurCkpt& k)
+{
+    mph_ctx* c = s->c;
+    const size_t n = c->n;
+    hipStream_t st = c->stream;
+    HIP_TRY(hipMemcpyAsync(c->pos.p, k.pos.p, n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->vel.p, k.vel.p, n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->vnext.p, k.vnext.p, n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->snap.p, k.snap.p, n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->pidx.p, k.pidx.p, n * sizeof(int), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->wallc.p, k.wallc.p, c->wallc.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    std::vector<int> ones(c->ncases, 1);
+    HIP_TRY(hipMemcpyAsync(c->flags.p, ones.data(), sizeof(int) * c->ncases, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->any_flag.p, ones.data(), sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));      // the host vector above must outlive the copies
+    for (int q = 0; q < c->ncases; ++q) c->cases[q].time = k.time[q];
+    c->steps = k.steps;
+    return MPH_OK;
+}
+
+extern "C" mph_surrogate* mph_sur_create(mph_ctx* c, const mph_sur_params* p)
+{
+    if (!c || !p) {
+        fail(MPH_ERR_ARG, "mph_sur_create: bad argument");
+        return nullptr;
+    }
+    if (c->ncases != 1 || c->mixed) {
+        fail(MPH_ERR_ARG, "the surrogate needs a single-case FP64 context");
+        return nullptr;
+    }
+    const CaseConst& k = c->cases[0].k;
+    for (int t = 2; t < kTypes; ++t)
+        for (int d = 0; d < 3; ++d)
+            if (k.wvel[t][d] != 0.0 || k.womega[t][d] != 0.0) {
+                fail(MPH_ERR_ARG, "the surrogate supports static walls only");
+                return nullptr;
+            }
+    if (!(p->u_scale > 0.0 && p->a_scale > 0.0 && p->train_stride >= 1)) {
+        fail(MPH_ERR_ARG, "mph_sur_create: scales must be positive and train_stride >= 1");
+        return nullptr;
+    }
+    std::unique_ptr<mph_surrogate> s(new mph_surrogate());
+    s->c = c;
+    s->prm = *p;
+    s->wh = c->width;
+    int rc = 0;
+    if ((rc = s->w.alloc(kSurNP)) || (rc = s->m1.alloc(kSurNP)) || (rc = s->m2.alloc(kSurNP)) ||
+        (rc = s->grad.alloc(kSurNP)) || (rc = s->wr.alloc(kSurNP)) || (rc = s->pk.alloc(kPkTiles * 64)) ||
+        (rc = s->pcnt.alloc(c->n)) || (rc = s->pnbr.alloc((size_t)c->n * s->wh)) || (rc = s->apred.alloc(c->n)) ||
+        (rc = s->loss.alloc(1)) || (rc = s->sums.alloc(4)) || (rc = s->v_orig.alloc(c->n)) ||
+        (rc = s->v_ref.alloc(c->n)) || (rc = s->n_ref.alloc(c->n)))
+        return nullptr;
+    // uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights from a splitmix64 stream, zero
+    // biases, output layer scaled by 0.1
+    std::vector<float> hw(kSurNP, 0.0f);
+    unsigned long long st = p->seed;
+    auto uni = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        unsigned long long z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+    };
+    for (int q = 0; q < 16 * kSurHid; ++q) hw[kOffW1 + q] = (float)(uni() / std::sqrt(16.0));
+    for (int q = 0; q < kSurHid * kSurHid; ++q) hw[kOffW2 + q] = (float)(uni() / 8.0);
+    for (int q = 0; q < kSurHid * kSurHid; ++q) hw[kOffW3 + q] = (float)(uni() / 8.0);
+    for (int q = 0; q < kSurHid * kSurOut; ++q) hw[kOffW4 + q] = (float)(0.1 * uni() / 8.0);
+    if (mph_sur_set_weights(s.get(), hw.data()) != MPH_OK) return nullptr;
+    return s.release();
+}
+
+extern "C" int mph_sur_set_weights(mph_surrogate* s, const float* w)
+{
+    if (!s || !w) return fail(MPH_ERR_ARG, "mph_sur_set_weights: bad argument");
+    hipStream_t st = s->c->stream;
+    HIP_TRY(hipMemcpyAsync(s->w.p, w, kSurNP * sizeof(float), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(s->m1.p, 0, kSurNP * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(s->m2.p, 0, kSurNP * sizeof(float), st));
+    s->t_adam = 0;
+    int rc = sur_pack(s);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    return MPH_OK;
+}
+
+extern "C" int mph_sur_get_weights(mph_surrogate* s, float* w)
+{
+    if (!s || !w) return fail(MPH_ERR_ARG, "mph_sur_get_weights: bad argument");
+    HIP_TRY(hipMemcpyAsync(w, s->w.p, kSurNP * sizeof(float), hipMemcpyDeviceToHost, s->c->stream));
+    HIP_TRY(hipStreamSynchronize(s->c->stream));
+    return MPH_OK;
+}
+
+extern "C" int mph_sur_predict(mph_surrogate* s, double* accel)
+{
+    if (!s || !accel) return fail(MPH_ERR_ARG, "mph_sur_predict: bad argument");
+    mph_ctx* c = s->c;
+    int rc = 0;
+    if ((rc = sur_pairs(s)) || (rc = sur_infer(s, c->vnext.p))) return rc;
+    std::vector<float4> h(c->n);
+    HIP_TRY(hipMemcpyAsync(h.data(), s->apred.p, sizeof(float4) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->n; ++i) {
+        accel[3 * i] = h[i].x;
+        accel[3 * i + 1] = h[i].y;
+        accel[3 * i + 2] = h[i].z;
+    }
+    return MPH_OK;
+}
+
+// training list of the current pair lists: fluid particles with sorted index % stride == 0,
+// with their tile offsets (the total rounded up to an even number of tiles)
+static int sur_make_list(mph_surrogate* s)
+{
+    mph_ctx* c = s->c;
+    std::vector<int> cnt(c->n), typ(c->n);
+    std::vector<double4> hp(c->n);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), s->pcnt.p, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hp.data(), c->pos.p, sizeof(double4) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<int> list, off, owner;
+    int tiles = 0;
+    for (int i = 0; i < c->n; ++i) {
+        long long tg;
+        std::memcpy(&tg, &hp[i].w, sizeof(tg));
+        if ((tg & 3) >= 2 || i % s->prm.train_stride) continue;
+        list.push_back(i);
+        off.push_back(tiles);
+        const int nt = (cnt[i] + 15) / 16;
+        for (int q = 0; q < nt; ++q) owner.push_back((int)list.size() - 1);
+        tiles += nt;
+    }
+    if (list.empty()) return fail(MPH_ERR_STATE, "no fluid particles to train on");
+    if (tiles & 1) {
+        owner.push_back((int)list.size() - 1);      // padding tile: all its slots are invalid
+        ++tiles;
+    }
+    s->nlist = (int)list.size();
+    s->ntiles = tiles;
+    s->nslots = 16l * tiles;
+    int rc = 0;
+    if ((size_t)s->nlist > s->plist.n &&
+        ((rc = s->plist.alloc(s->nlist)) || (rc = s->toff.alloc(s->nlist)) || (rc = s->target.alloc(s->nlist)) ||
+         (rc = s->dmk.alloc(s->nlist))))
+        return rc;
+    if ((size_t)tiles > s->towner.n && (rc = s->towner.alloc(tiles))) return rc;
+    if (s->nslots > s->cap_slots) {
+        const long cap = s->nslots + s->nslots / 4;
+        if ((rc = s->X.alloc((size_t)cap * 32)) || (rc = s->Z1.alloc((size_t)cap * 64)) ||
+            (rc = s->Z2.alloc((size_t)cap * 64)) || (rc = s->Z3.alloc((size_t)cap * 64)) ||
+            (rc = s->dz1.alloc((size_t)cap * 64)) || (rc = s->dz2.alloc((size_t)cap * 64)) ||
+            (rc = s->dz3.alloc((size_t)cap * 64)) || (rc = s->dm.alloc((size_t)cap * 4)))
+            return rc;
+        s->cap_slots = cap;
+        // slots of the padding tile are never written by the forward: start from zeros
+        HIP_TRY(hipMemsetAsync(s->X.p, 0, sizeof(__bf16) * cap * 32, c->stream));
+        HIP_TRY(hipMemsetAsync(s->Z1.p, 0, sizeof(float) * cap * 64, c->stream));
+        HIP_TRY(hipMemsetAsync(s->Z2.p, 0, sizeof(float) * cap * 64, c->stream));
+        HIP_TRY(hipMemsetAsync(s->Z3.p, 0, sizeof(float) * cap * 64, c->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(s->plist.p, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(s->toff.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(s->towner.p, owner.data(), sizeof(int) * owner.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MPH_OK;
+}
+
+// forward (storing activations), loss, backward and weight gradients over the training
+// list at (pos, velf, strain) with s->target set; gradient in s->grad, loss in s->loss
+static int sur_fwd_bwd(mph_surrogate* s, const double4* pos, const double4* velf, const double* strain)
+{
+    mph_ctx* c = s->c;
+    hipStream_t st = c->stream;
+    HIP_TRY(hipMemsetAsync(s->loss.p, 0, sizeof(double), st));
+    HIP_TRY(hipMemsetAsync(s->grad.p, 0, sizeof(float) * kSurNP, st));
+    hipLaunchKernelGGL((k_sur_forward<true>), dim3(sur_grid(c)), dim3(256), 0, st, c->d_cases.p, c->n, s->plist.p,
+                       s->nlist, s->toff.p, pos, velf, strain, c->pidx.p, s->pcnt.p, s->pnbr.p, s->wh, s->pk.p,
+                       s->w.p, sur_scales(s), s->apred.p, s->X.p, s->Z1.p, s->Z2.p, s->Z3.p);
+    hipLaunchKernelGGL(k_sur_loss, dim3((s->nlist + 255) / 256), dim3(256), 0, st, s->plist.p, s->nlist, s->apred.p,
+                       s->target.p, (float)(1.0 / s->prm.a_scale), s->dmk.p, s->loss.p);
+    const unsigned gb = (unsigned)((s->nslots + 255) / 256);
+    hipLaunchKernelGGL((k_sur_backward<kSurOut, true>), dim3(gb), dim3(256), 0, st, s->nslots, s->towner.p, s->toff.p,
+                       s->plist.p, s->pcnt.p, s->dmk.p, s->wr.p, kOffW4, nullptr, s->Z3.p, s->dz3.p, s->dm.p);
+    hipLaunchKernelGGL((k_sur_backward<kSurHid, false>), dim3(gb), dim3(256), 0, st, s->nslots, s->towner.p,
+                       s->toff.p, s->plist.p, s->pcnt.p, s->dmk.p, s->wr.p, kOffW3, s->dz3.p, s->Z2.p, s->dz2.p,
+                       nullptr);
+    hipLaunchKernelGGL((k_sur_backward<kSurHid, false>), dim3(gb), dim3(256), 0, st, s->nslots, s->towner.p,
+                       s->toff.p, s->plist.p, s->pcnt.p, s->dmk.p, s->wr.p, kOffW2, s->dz2.p, s->Z1.p, s->dz1.p,
+                       nullptr);
+    const long nslices = s->nslots / 32;
+    const int waves = 1024;
+    const int per = (int)((nslices + waves - 1) / waves);
+    const unsigned gw = (unsigned)((nslices + per - 1) / per + 3) / 4;
+    hipLaunchKernelGGL((k_sur_wgrad<0, 2, 4>), dim3(gw), dim3(256), 0, st, s->nslots, per, s->X.p, nullptr, s->dz1.p,
+                       kSurHid, kSurHid, kSurIn, s->grad.p, kOffW1, -1);
+    hipLaunchKernelGGL((k_sur_wgrad<1, 4, 4>), dim3(gw), dim3(256), 0, st, s->nslots, per, nullptr, s->Z1.p, s->dz2.p,
+                       kSurHid, kSurHid, kSurHid, s->grad.p, kOffW2, kOffB2);
+    hipLaunchKernelGGL((k_sur_wgrad<1, 4, 4>), dim3(gw), dim3(256), 0, st, s->nslots, per, nullptr, s->Z2.p, s->dz3.p,
+                       kSurHid, kSurHid, kSurHid, s->grad.p, kOffW3, kOffB3);
+    hipLaunchKernelGGL((k_sur_wgrad<1, 4, 1>), dim3(gw), dim3(256), 0, st, s->nslots, per, nullptr, s->Z3.p, s->dm.p, 4,
+                       kSurOut, kSurHid, s->grad.p, kOffW4, kOffB4);
+    return launch_ok("surrogate backward");
+}
+
+static int sur_adam(mph_surrogate* s)
+{
+    ++s->t_adam;
+    const double b1 = s->prm.beta1, b2 = s->prm.beta2;
+    const float c1 = (float)(1.0 / (1.0 - std::pow(b1, (double)s->t_adam)));
+    const float c2 = (float)(1.0 / (1.0 - std::pow(b2, (double)s->t_adam)));
+    hipLaunchKernelGGL(k_sur_adam, dim3((kSurNP + 255) / 256), dim3(256), 0, s->c->stream, s->w.p, s->m1.p, s->m2.p,
+                       s->grad.p, (float)s->prm.lr, (float)b1, (float)b2, (float)s->prm.eps, c1, c2);
+    int rc = launch_ok("adam");
+    return rc ? rc : sur_pack(s);
+}
+
+extern "C" int mph_sur_gradient(mph_surrogate* s, const double* target, double* loss, float* grad)
+{
+    if (!s || !target || !loss) return fail(MPH_ERR_ARG, "mph_sur_gradient: bad argument");
+    mph_ctx* c = s->c;
+    int rc = 0;
+    if ((rc = sur_pairs(s)) || (rc = sur_make_list(s))) return rc;
+    std::vector<int> pidx(c->n), list(s->nlist);
+    HIP_TRY(hipMemcpyAsync(pidx.data(), c->pidx.p, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(list.data(), s->plist.p, sizeof(int) * s->nlist, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<float4> t(s->nlist);
+    for (int k = 0; k < s->nlist; ++k) {
+        const int o = pidx[list[k]];
+        t[k] = make_float4((float)target[3 * o], (float)target[3 * o + 1], (float)target[3 * o + 2], 0.f);
+    }
+    HIP_TRY(hipMemcpyAsync(s->target.p, t.data(), sizeof(float4) * t.size(), hipMemcpyHostToDevice, c->stream));
+    if ((rc = sur_fwd_bwd(s, c->pos.p, c->vnext.p, c->vstrain.p))) return rc;
+    HIP_TRY(hipMemcpyAsync(loss, s->loss.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (grad) HIP_TRY(hipMemcpyAsync(grad, s->grad.p, sizeof(float) * kSurNP, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MPH_OK;
+}
+
+extern "C" int mph_sur_train(mph_surrogate* s, int samples, int iters, double* losses)
+{
+    if (!s || samples < 0 || iters < 0) return fail(MPH_ERR_ARG, "mph_sur_train: bad argument");
+    mph_ctx* c = s->c;
+    hipStream_t st = c->stream;
+    const size_t n = c->n;
+    int rc = 0;
+    if (!s->spos.p && ((rc = s->spos.alloc(n)) || (rc = s->svel.alloc(n)) || (rc = s->sdens.alloc(n)) ||
+                       (rc = s->spidx.alloc(n))))
+        return rc;
+    for (int smp = 0; smp < samples; ++smp) {
+        // plain step n+1: state (x_{n+1}, v_n) and volumetric strain at x_{n+1}
+        if ((rc = launch_step(c))) return rc;
+        for (auto& ch : c->cases) ch.time += ch.prm.dt;
+        ++c->steps;
+        if ((rc = sur_pairs(s)) || (rc = sur_make_list(s))) return rc;
+        HIP_TRY(hipMemcpyAsync(s->spos.p, c->pos.p, n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(s->svel.p, c->vel.p, n * sizeof(double4), hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(s->sdens.p, c->vstrain.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(s->spidx.p, c->pidx.p, n * sizeof(int), hipMemcpyDeviceToDevice, st));
+        // plain step n+2: v_{n+2}, by original id
+        if ((rc = launch_step(c))) return rc;
+        for (auto& ch : c->cases) ch.time += ch.prm.dt;
+        ++c->steps;
+        hipLaunchKernelGGL(k_sur_to_orig, dim3((c->n + 255) / 256), dim3(256), 0, st, c->n, c->pidx.p, c->vnext.p,
+                           nullptr, s->v_orig.p, nullptr);
+        hipLaunchKernelGGL(k_sur_target, dim3((s->nlist + 255) / 256), dim3(256), 0, st, c->d_cases.p, s->plist.p,
+                           s->nlist, s->spidx.p, s->svel.p, s->v_orig.p, s->target.p);
+        if ((rc = launch_ok("surrogate targets"))) return rc;
+        for (int it = 0; it < iters; ++it) {
+            // the snapshot's sorted order is the one its pair lists and training list use
+            if ((rc = sur_fwd_bwd(s, s->spos.p, s->svel.p, s->sdens.p)) || (rc = sur_adam(s))) return rc;
+            if (losses) {
+                HIP_TRY(hipMemcpyAsync(&losses[(size_t)smp * iters + it], s->loss.p, sizeof(double),
+                                       hipMemcpyDeviceToHost, st));
+            }
+        }
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    return check_capacity(c);
+}
+
+extern "C" int mph_sur_step(mph_surrogate* s, int nsteps)
+{
+    if (!s || nsteps < 0) return fail(MPH_ERR_ARG, "mph_sur_step: bad argument");
+    for (int k = 0; k < nsteps; ++k) {
+        int rc = sur_step(s);
+        if (rc) return rc;
+    }
+    return check_capacity(s->c);
+}
+
+static int plain_steps(mph_ctx* c, int k)
+{
+    for (int q = 0; q < k; ++q) {
+        int rc = launch_step(c);
+        if (rc) return rc;
+        for (auto& ch : c->cases) ch.time += ch.prm.dt;
+        ++c->steps;
+    }
+    return MPH_OK;
+}
+
+extern "C" int mph_sur_advance(mph_surrogate* s, int blocks, int k, double tol, mph_sur_stats* out)
+{
+    if (!s || blocks < 0 || k < 1 || !(tol > 0.0) || !out) return fail(MPH_ERR_ARG, "mph_sur_advance: bad argument");
+    mph_ctx* c = s->c;
+    hipStream_t st = c->stream;
+    const double t0 = c->cases[0].time;
+    int rc = 0;
+    for (int b = 0; b < blocks; ++b) {
+        if ((rc = sur_ckpt_save(s, s->ck[0]))) return rc;
+        for (int q = 0; q < k; ++q)
+            if ((rc = sur_step(s))) return rc;
+        // judge: one surrogate step and two plain steps from the same state
+        if ((rc = sur_ckpt_save(s, s->ck[1])) || (rc = sur_step(s)) || (rc = launch_sweep_a(c, 0))) return rc;
+        hipLaunchKernelGGL(k_sur_to_orig, dim3((c->n + 255) / 256), dim3(256), 0, st, c->n, c->pidx.p, c->vnext.p,
+                           c->dens_a.p, s->v_ref.p, s->n_ref.p);
+        if ((rc = launch_ok("judge")) || (rc = sur_ckpt_load(s, s->ck[1])) || (rc = plain_steps(c, 2))) return rc;
+        HIP_TRY(hipMemsetAsync(s->sums.p, 0, 4 * sizeof(double), st));
+        hipLaunchKernelGGL(k_sur_compare, dim3((c->n + 255) / 256), dim3(256), 0, st, c->n, c->pos.p, c->pidx.p,
+                           c->vnext.p, c->dens_a.p, s->v_ref.p, s->n_ref.p, s->sums.p);
+        double h[4];
+        HIP_TRY(hipMemcpyAsync(h, s->sums.p, sizeof(h), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        const double ev = std::sqrt(h[0] / std::max(h[1], 1e-300));
+        const double en = std::sqrt(h[2] / std::max(h[3], 1e-300));
+        out->blocks += 1;
+        out->last_err_v = ev;
+        out->last_err_n = en;
+        out->max_err_v = std::max(out->max_err_v, ev);
+        out->max_err_n = std::max(out->max_err_n, en);
+        if (ev > tol || en > tol || !(ev == ev) || !(en == en)) {
+            // roll back the block and replay it with plain steps
+            if ((rc = sur_ckpt_load(s, s->ck[0])) || (rc = plain_steps(c, 2 * k))) return rc;
+            out->rollbacks += 1;
+            out->plain_steps += 2 * k;
+        } else {
+            out->sur_steps += k;
+            out->plain_steps += 2;
+        }
+    }
+    out->sim_time += c->cases[0].time - t0;
+    return check_capacity(c);
+}
+
+extern "C" void mph_sur_destroy(mph_surrogate* s)
+{
+    // touches no context state: the context may already be destroyed (hipFree synchronises)
+    delete s;
 }

@@ -12,7 +12,8 @@ from typing import Dict, Iterable, List, Sequence
 
 import numpy as np
 
-from .abi import FIELD_SHAPES, MphFields, MphParams, MphStats, MphTiming, params_from_deck
+from .abi import (FIELD_SHAPES, SUR_NPARAMS, MphFields, MphParams, MphStats, MphSurParams, MphSurStats,
+                  MphTiming, params_from_deck)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmph_hip.so")
 
@@ -59,6 +60,17 @@ def load_library() -> C.CDLL:
         "mph_get_stats": (C.c_int, [vp, C.POINTER(MphStats)]),
         "mph_destroy": (None, [vp]),
         "mph_last_error": (C.c_char_p, []),
+        "mph_set_time_step": (C.c_int, [vp, C.c_int, C.c_double]),
+        "mph_sur_params_default": (None, [C.POINTER(MphSurParams)]),
+        "mph_sur_create": (vp, [vp, C.POINTER(MphSurParams)]),
+        "mph_sur_get_weights": (C.c_int, [vp, vp]),
+        "mph_sur_set_weights": (C.c_int, [vp, vp]),
+        "mph_sur_predict": (C.c_int, [vp, vp]),
+        "mph_sur_gradient": (C.c_int, [vp, vp, dp, vp]),
+        "mph_sur_train": (C.c_int, [vp, C.c_int, C.c_int, vp]),
+        "mph_sur_step": (C.c_int, [vp, C.c_int]),
+        "mph_sur_advance": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.POINTER(MphSurStats)]),
+        "mph_sur_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -131,6 +143,9 @@ class Solver:
         _check(self.lib.mph_step_timed(self.h, n, C.byref(t)))
         return {k: getattr(t, k) for k, _ in MphTiming._fields_}
 
+    def set_dt(self, dt: float, case: int = 0) -> None:
+        _check(self.lib.mph_set_time_step(self.h, case, dt))
+
     def sync(self) -> None:
         _check(self.lib.mph_sync(self.h))
 
@@ -180,6 +195,74 @@ class Solver:
 
     def __exit__(self, *a):
         self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sur_params(**kw) -> MphSurParams:
+    """mph_sur_params with the library defaults, overridden by keyword"""
+    p = MphSurParams()
+    load_library().mph_sur_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Surrogate:
+    """The pairwise-message surrogate bound to a single-case Solver (mph_sur_* of mph.h)."""
+
+    def __init__(self, solver: "Solver", **params):
+        self.lib = load_library()
+        self.solver = solver
+        self.params = sur_params(**params)
+        h = self.lib.mph_sur_create(solver.h, C.byref(self.params))
+        if not h:
+            raise MphError(-1, self.lib.mph_last_error().decode())
+        self.h = h
+
+    def weights(self) -> np.ndarray:
+        w = np.zeros(SUR_NPARAMS, np.float32)
+        _check(self.lib.mph_sur_get_weights(self.h, w.ctypes.data))
+        return w
+
+    def set_weights(self, w: np.ndarray) -> None:
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.shape == (SUR_NPARAMS,)
+        _check(self.lib.mph_sur_set_weights(self.h, w.ctypes.data))
+
+    def predict(self) -> np.ndarray:
+        a = np.zeros((int(self.solver.counts[0]), 3))
+        _check(self.lib.mph_sur_predict(self.h, a.ctypes.data))
+        return a
+
+    def gradient(self, target: np.ndarray):
+        t = np.ascontiguousarray(target, np.float64)
+        loss = C.c_double(0.0)
+        g = np.zeros(SUR_NPARAMS, np.float32)
+        _check(self.lib.mph_sur_gradient(self.h, t.ctypes.data, C.byref(loss), g.ctypes.data))
+        return loss.value, g
+
+    def train(self, samples: int, iters: int) -> np.ndarray:
+        losses = np.zeros(max(samples * iters, 1))
+        _check(self.lib.mph_sur_train(self.h, samples, iters, losses.ctypes.data))
+        return losses[: samples * iters]
+
+    def step(self, n: int = 1) -> None:
+        _check(self.lib.mph_sur_step(self.h, n))
+
+    def advance(self, blocks: int, k: int = 10, tol: float = 0.01) -> dict:
+        st = MphSurStats()
+        _check(self.lib.mph_sur_advance(self.h, blocks, k, tol, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in MphSurStats._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.mph_sur_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:

@@ -1,0 +1,159 @@
+"""The pairwise-message surrogate on the GPU (mph_sur_* through the C ABI):
+
+* forward (BF16 matrix cores, FP32 accumulation) against the CPU oracle with the same
+  bfloat16 roundings (oracle/surrogate.py), on a 3D dam-break state;
+* loss and gradient of the project's backward kernels against the oracle's analytic
+  gradient (the oracle's gradient is pinned by finite differences in
+  test_surrogate_oracle.py);
+* training on the solver's own trajectory lowers the loss;
+* the judge rolls back an untrained network and keeps a trained one;
+* what the trained network gains: 2 dt surrogate steps against plain steps at 2 dt and the
+  untrained network, all measured against the plain FP64 run at dt.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+from mphexplicit_amd import decks
+from oracle import surrogate as S
+from oracle.oracle import Oracle
+from tests.util import normwise
+
+pytestmark = pytest.mark.gpu
+
+SCALES = dict(u_scale=0.1, a_scale=10.0, dens_scale=50.0)
+
+
+@pytest.fixture(scope="module")
+def mods(gpu_available):
+    if not gpu_available:
+        pytest.skip("no GPU")
+    from mphexplicit_amd import solver
+    return solver
+
+
+def _oracle_inputs(deck, s):
+    f = s.fields(names=("position", "velocity", "vol_strain_p", "type"))
+    ptr, ind = s.neighbors()
+    sc = Oracle(deck).scalars()
+    n0p, maxr = sc[1], sc[3]
+    lo, hi = deck.domain_min, deck.domain_max
+    pi, pj = S.pairs_within(f, ptr, ind, lo, hi, maxr)
+    X = S.features(f, pi, pj, lo, hi, maxr, n0p, SCALES["u_scale"], SCALES["dens_scale"])
+    return f, pi, X
+
+
+def _state(mods, steps=6):
+    d = decks.small_dam_break_3d()
+    s = mods.Solver.from_deck(d)
+    s.step(steps)
+    return d, s
+
+
+def test_forward_matches_oracle(mods):
+    d, s = _state(mods)
+    sur = mods.Surrogate(s, seed=7, **SCALES)
+    w = sur.weights()
+    a_gpu = sur.predict()
+    f, pi, X = _oracle_inputs(d, s)
+    a_orc = S.predict(w, X, pi, d.n, SCALES["a_scale"], emulate=True)
+    fl = f["type"] < 2
+    err = normwise(a_gpu[fl], a_orc[fl])
+    print("surrogate forward normwise vs oracle %.2e, pairs %d" % (err, len(pi)), flush=True)
+    # same bf16 roundings; differences come from FP32 vs FP64 arithmetic before a rounding
+    assert err <= 2e-3
+    assert np.all(a_gpu[~fl] == 0.0)
+
+
+def test_gradient_matches_oracle(mods):
+    d, s = _state(mods)
+    sur = mods.Surrogate(s, seed=11, **SCALES)
+    w = sur.weights()
+    rng = np.random.default_rng(3)
+    target = rng.normal(size=(d.n, 3)) * 5.0
+    loss_g, g_gpu = sur.gradient(target)
+    f, pi, X = _oracle_inputs(d, s)
+    train = np.nonzero(f["type"] < 2)[0]
+    loss_o, g_orc = S.loss_and_grad(w, X, pi, train, target, SCALES["a_scale"], emulate=True)
+    print("loss gpu %.8e oracle %.8e" % (loss_g, loss_o), flush=True)
+    assert abs(loss_g - loss_o) <= 1e-4 * loss_o
+    o = 0
+    for name, shp in S.SHAPES:
+        m = int(np.prod(shp))
+        err = normwise(g_gpu[o:o + m], g_orc[o:o + m])
+        print("grad %s normwise %.2e" % (name, err), flush=True)
+        # bf16 inputs of the weight-gradient products (emulated by the oracle), FP32 sums
+        assert err <= 1e-2, name
+        o += m
+
+
+def _trained_weights(mods, deck, samples=40, iters=25):
+    s = mods.Solver.from_deck(deck)
+    sur = mods.Surrogate(s, seed=5, lr=2e-3, **SCALES)
+    losses = sur.train(samples, iters)
+    w = sur.weights()
+    sur.close()
+    s.close()
+    return w, losses
+
+
+def test_training_lowers_loss(mods):
+    w, losses = _trained_weights(mods, decks.small_dam_break_3d(), samples=20, iters=20)
+    first, last = losses[:5].mean(), losses[-20:].mean()
+    print("training loss first %.4e last %.4e" % (first, last), flush=True)
+    assert np.all(np.isfinite(losses))
+    assert last < 0.3 * first
+
+
+def _errors(a, b):
+    fl = b["type"] < 2
+    ev = np.linalg.norm(a["velocity"][fl] - b["velocity"][fl]) / np.linalg.norm(b["velocity"][fl])
+    en = np.linalg.norm(a["density_a"][fl] - b["density_a"][fl]) / np.linalg.norm(b["density_a"][fl])
+    return float(ev), float(en)
+
+
+def test_judge_and_gain(mods):
+    deck = decks.small_dam_break_3d()
+    w, losses = _trained_weights(mods, deck, samples=60, iters=20)
+    print("training loss first %.4e last %.4e" % (losses[:5].mean(), losses[-20:].mean()), flush=True)
+    names = ("velocity", "density_a", "type")
+    W = 100                                 # plain steps before the comparison (flow developed)
+    M = 40                                  # surrogate steps = 2 M plain steps of dt
+
+    def start(d=deck, steps=W):
+        s = mods.Solver.from_deck(d)
+        s.step(steps)
+        return s
+
+    # plain FP64 at dt: the reference trajectory
+    ref = start()
+    ref.step(2 * M)
+    fr = ref.fields(names=names)
+    # without the network: plain steps at 2 dt from the same state
+    p2 = start()
+    p2.set_dt(2.0 * deck.data.dt)
+    p2.step(M)
+    results = {"plain_2dt": _errors(p2.fields(names=names), fr)}
+    for label, weights in (("untrained", None), ("trained", w)):
+        s = start()
+        sur = mods.Surrogate(s, seed=5, **SCALES)
+        if weights is not None:
+            sur.set_weights(weights)
+        sur.step(M)
+        assert abs(s.time() - ref.time()) <= 1e-12 * ref.time()
+        results[label] = _errors(s.fields(names=names), fr)
+        s2 = start()
+        sur2 = mods.Surrogate(s2, seed=5, **SCALES)
+        if weights is not None:
+            sur2.set_weights(weights)
+        st = sur2.advance(blocks=3, k=10, tol=0.01)
+        results[label + "_judged"] = st
+        print(label, "judged", st, flush=True)
+    print("relative L2 (velocity, density) after %d dt from step %d: %s" % (
+        2 * M, W, {k: v for k, v in results.items() if not k.endswith("judged")}), flush=True)
+    tr, un = results["trained"], results["untrained"]
+    # the trained network is closer to the plain run than the untrained one
+    assert tr[0] < un[0] and tr[1] < un[1]
+    # the judge rejects a block whose one-step error is above the bound
+    assert results["untrained_judged"]["rollbacks"] >= 1
