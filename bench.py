@@ -118,6 +118,45 @@ def batch_leg(args, Solver):
                          "sweep_a": tm["sweep_a_ms"] / k2, "sweep_b": tm["sweep_b_ms"] / k2}}
 
 
+SUR_SCALES = dict(u_scale=0.1, a_scale=10.0, dens_scale=50.0)
+
+
+def surrogate_leg(args, deck, plain_sps):
+    """Learned 2 dt steps on the same deck: train on the deck's own plain trajectory (from a
+    developed flow), then time bare surrogate steps and judged blocks (K surrogate steps,
+    one-step judge against two plain steps, rollback above the bound)."""
+    from mphexplicit_amd.solver import Solver, Surrogate
+    s = Solver.from_deck(deck)
+    s.step(args.sur_warm)
+    sur = Surrogate(s, train_stride=args.sur_stride, lr=2e-3, seed=5, **SUR_SCALES)
+    s.sync()
+    t0 = time.perf_counter()
+    losses = sur.train(args.sur_samples, args.sur_iters)
+    t_train = time.perf_counter() - t0
+    sur.step(2)
+    s.sync()
+    t0 = time.perf_counter()
+    sur.step(args.sur_steps)
+    s.sync()
+    t_bare = (time.perf_counter() - t0) / args.sur_steps
+    t0 = time.perf_counter()
+    st = sur.advance(args.sur_blocks, args.sur_k, args.sur_tol)
+    t_judged = time.perf_counter() - t0
+    dt = deck.data.dt
+    plain_sim = dt * plain_sps
+    out = {"train_s": t_train, "train_samples": args.sur_samples, "train_iters": args.sur_iters,
+           "train_stride": args.sur_stride, "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+           "ms_per_surrogate_step": 1e3 * t_bare, "ms_per_two_plain_steps": 2e3 / plain_sps,
+           "bare_sim_seconds_per_s": 2.0 * dt / t_bare,
+           "judged": st, "judged_sim_seconds_per_s": st["sim_time"] / t_judged,
+           "plain_sim_seconds_per_s": plain_sim, "tol": args.sur_tol, "k": args.sur_k}
+    out["judged_vs_plain"] = out["judged_sim_seconds_per_s"] / plain_sim
+    out["bare_vs_plain"] = out["bare_sim_seconds_per_s"] / plain_sim
+    sur.close()
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +169,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-case batch leg")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-step leg")
+    ap.add_argument("--sur-warm", type=int, default=100)
+    ap.add_argument("--sur-samples", type=int, default=20)
+    ap.add_argument("--sur-iters", type=int, default=10)
+    ap.add_argument("--sur-stride", type=int, default=32)
+    ap.add_argument("--sur-steps", type=int, default=10)
+    ap.add_argument("--sur-blocks", type=int, default=2)
+    ap.add_argument("--sur-k", type=int, default=10)
+    ap.add_argument("--sur-tol", type=float, default=0.01)
     ap.add_argument("--error-steps", type=int, default=500,
                     help="steps after which the mixed leg's fields are compared with FP64")
     args = ap.parse_args()
@@ -182,6 +230,9 @@ def main():
                  "error_vs_fp64_after_steps": n_err, "error_vs_fp64": errs}
         m.close()
     batch = None if args.no_batch else batch_leg(args, Solver)
+    surrogate = None
+    if prec == MPH_FP64 and not args.no_surrogate:
+        surrogate = surrogate_leg(args, deck, steps_per_s)
     cpu = None
     if not args.no_cpu_baseline:
         cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
@@ -215,6 +266,7 @@ def main():
         "cpu_baseline": cpu,
         "mixed": mixed,
         "batch64": batch,
+        "surrogate": surrogate,
     }
     if cpu:
         out["speedup_vs_cpu"] = steps_per_s / cpu["value"]

@@ -722,8 +722,11 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
     walk_union_lds<K, 2>(f, lds, ldsj, fetch, eval);
 }
 
+// without surface tension sweep A fits six wavefronts per SIMD (80 VGPRs, a few spilled
+// outside the pair loop), which measures faster than five; the other sweeps keep the
+// compiler's allocation
 template <int SURF, int K>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 6, 8)))
 k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
           const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
