@@ -64,6 +64,21 @@ def cpu_baseline(deck, steps: int, cores: int):
                       % (steps + 1, deck.n, cores, rb)}
 
 
+def measured_traffic(workload: str, cluster_size: int, kernel: str):
+    """HBM bytes per launch of a sweep from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    (profiles/hbm_traffic_<workload>.json), if they were taken on the same configuration"""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s.json" % workload)
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload or t.get("cluster_size") != cluster_size:
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return float(k["bytes"]) if k else None
+
+
 def timed_run(s, args):
     """warmup, timed steps (wall clock around the stream), then a HIP-event window"""
     stepper = s.step_graph if args.graph else s.step
@@ -205,10 +220,14 @@ def main():
     fl = (fa if dom == "sweep_a" else fb) * pairs_per_step
     achieved = fl / (phases[dom] * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_" + dom,
+                "frac": achieved / FP64_PEAK_TFLOPS,
+                "traffic": measured_traffic(deck.name, st["cluster_size"], "k_" + dom) if prec == MPH_FP64 else None,
+                "kernel": "k_" + dom,
                 "note": "FP64 VALU kernel; MI355X's fp64 vector and matrix peaks are both 78.6 TFLOP/s; "
-                        "achieved = %d algorithmic flops/pair x %.4g pairs per launch / mean launch time"
-                        % (fa if dom == "sweep_a" else fb, pairs_per_step)}
+                        "achieved = %d algorithmic flops/pair x %.4g pairs per launch / mean launch time; "
+                        "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a "
+                        "copy probe), profiles/hbm_traffic_%s.json" % (fa if dom == "sweep_a" else fb,
+                                                                      pairs_per_step, deck.name)}
     mixed = None
     if prec == MPH_FP64 and not args.no_mixed:
         # the same deck with FP32 pair math: timing, and the error of every field against the

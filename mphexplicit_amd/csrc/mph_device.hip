@@ -516,8 +516,8 @@ static int prepare_clusters(mph_ctx* c, int K)
 // ---------------------------------------------------------------------------------------
 // autotuner: time the sweeps for every cluster size on the context's own particles and
 // keep the fastest.  Results are cached in a text file (MPH_TUNE_CACHE, default
-// ~/.cache/mphexplicit_amd/cluster_size.txt) keyed by device, dimension, surface tension
-// and problem size; MPH_TUNE=0 keeps K = 4, MPH_TUNE=<2|4|8> forces a size and
+// ~/.cache/mphexplicit_amd/cluster_size.txt) keyed by device, dimension, surface tension,
+// precision, case count and problem size; MPH_TUNE=0 keeps K = 4, MPH_TUNE=<2|4|8> forces a size and
 // MPH_TUNE=force ignores the cache.
 
 static std::string tune_key(mph_ctx* c)
