@@ -167,3 +167,11 @@ MPHREF_API void mphref_virial(double* vp /* [n] */)
     calculateVirialStressAtParticle();
     memcpy(vp, VirialPressureAtParticle, sizeof(double) * ParticleCount);
 }
+
+// the reference's own main() (main.cpp:290-470): reads the files named on its command line,
+// runs the whole time loop and writes its .prof/.vtk/log outputs in the working directory
+MPHREF_API int mphref_main(int argc, char** argv)
+{
+    reset_globals();
+    return mphref_reference_main(argc, argv);
+}
