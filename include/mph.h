@@ -208,16 +208,19 @@ const char *mph_last_error(void);
  * Learned part: a pairwise-message surrogate.  No counterpart in the reference.
  *
  * For each fluid particle i and each neighbour j within the largest interaction radius h
- * (the reference's neighbour set, main.cpp:1408), a 17-input MLP (features below; three
- * hidden layers of 64 SiLU units; 3 outputs) gives a message m_ij, and the predicted mean
- * acceleration over two steps is a_i = a_scale * sum_j m_ij.  Weights are BF16 on the
- * matrix cores with FP32 accumulation; master weights and Adam state are FP32.
+ * (the reference's neighbour set, main.cpp:1408), a 17-input MLP (three hidden layers of 64
+ * SiLU units, 3 outputs c0..c2) gives the coefficients of an antisymmetric message
+ *   m_ij = c0 e + c1 u + c2 (u.e) e,   e = x_ij / r,   u = (v_j - v_i) / u_scale,
+ * and the predicted mean acceleration over two steps is a_i = a_scale (rho_0/rho_i) sum_j m_ij.
+ * All inputs are symmetric under i <-> j, so m_ji = -m_ij: pairwise momentum exchange is
+ * built in.  Weights are BF16 on the matrix cores with FP32 accumulation; master weights and
+ * Adam state are FP32.
  *
- * Features of a pair (x_ij = periodic x_j - x_i, r = |x_ij|, u = (v_j - v_i)/u_scale):
- *   0-2 x_ij/r   3 r/h   4-6 u   7 u.(x_ij/r)   8-10 class of i   11-13 class of j
- *   (one-hot: type 0, type 1, wall)   14 dens_scale e^P_i / N0p, the density deviation that
- *   drives the pressure (VolStrainP = sum w_P - N0p, main.cpp:1850-1856)   15 the same for j
- *   16 the constant 1 (carries the first layer's bias)
+ * Inputs (x_ij = periodic x_j - x_i, r = |x_ij|, eps = dens_scale e^P / N0p with the
+ * volumetric strain VolStrainP = sum w_P - N0p, main.cpp:1850-1856):
+ *   0 r/h   1 (r/h)^2   2 u.e   3 |u|^2   4-9 one-hot of the unordered class pair
+ *   (00, 01, 02, 11, 12, 22; class = type 0, type 1, wall)   10 eps_i + eps_j
+ *   11 (eps_i - eps_j)^2   12 eps_i eps_j   13-15 zero   16 the constant 1 (bias of layer 1)
  *
  * A surrogate step advances 2 dt: the plain move to x_{n+1}, sweep A's densities at x_{n+1},
  * the MLP at (x_{n+1}, v_n), v_{n+1} = v_n + dt (a + g), the plain move to x_{n+2} and

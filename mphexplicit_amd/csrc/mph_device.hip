@@ -1062,7 +1062,7 @@ struct mph_surrogate {
     int nlist = 0, ntiles = 0;
     long nslots = 0, cap_slots = 0;
     DevBuf<__bf16> X;
-    DevBuf<float> Z1, Z2, Z3, dz1, dz2, dz3, dm;
+    DevBuf<float> Z1, Z2, Z3, dz1, dz2, dz3, dm, Bs;
     DevBuf<float4> target, dmk;
     DevBuf<double> loss, sums;
     // snapshot of a training state and original-order scratch
@@ -1135,7 +1135,7 @@ static int sur_infer(mph_surrogate* s, const double4* velf)
     mph_ctx* c = s->c;
     hipLaunchKernelGGL((k_sur_forward<false>), dim3(sur_grid(c)), dim3(256), 0, c->stream, c->d_cases.p, c->n,
                        nullptr, 0, nullptr, c->pos.p, velf, c->vstrain.p, c->pidx.p, s->pcnt.p, s->pnbr.p, s->wh,
-                       s->pk.p, s->w.p, sur_scales(s), s->apred.p, nullptr, nullptr, nullptr, nullptr);
+                       s->pk.p, s->w.p, sur_scales(s), s->apred.p, nullptr, nullptr, nullptr, nullptr, nullptr);
     return launch_ok("surrogate forward");
 }
 
@@ -1335,7 +1335,8 @@ static int sur_make_list(mph_surrogate* s)
         if ((rc = s->X.alloc((size_t)cap * 32)) || (rc = s->Z1.alloc((size_t)cap * 64)) ||
             (rc = s->Z2.alloc((size_t)cap * 64)) || (rc = s->Z3.alloc((size_t)cap * 64)) ||
             (rc = s->dz1.alloc((size_t)cap * 64)) || (rc = s->dz2.alloc((size_t)cap * 64)) ||
-            (rc = s->dz3.alloc((size_t)cap * 64)) || (rc = s->dm.alloc((size_t)cap * 4)))
+            (rc = s->dz3.alloc((size_t)cap * 64)) || (rc = s->dm.alloc((size_t)cap * 4)) ||
+            (rc = s->Bs.alloc((size_t)cap * 8)))
             return rc;
         s->cap_slots = cap;
         // slots of the padding tile are never written by the forward: start from zeros
@@ -1343,6 +1344,7 @@ static int sur_make_list(mph_surrogate* s)
         HIP_TRY(hipMemsetAsync(s->Z1.p, 0, sizeof(float) * cap * 64, c->stream));
         HIP_TRY(hipMemsetAsync(s->Z2.p, 0, sizeof(float) * cap * 64, c->stream));
         HIP_TRY(hipMemsetAsync(s->Z3.p, 0, sizeof(float) * cap * 64, c->stream));
+        HIP_TRY(hipMemsetAsync(s->Bs.p, 0, sizeof(float) * cap * 8, c->stream));
     }
     HIP_TRY(hipMemcpyAsync(s->plist.p, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(s->toff.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, c->stream));
@@ -1361,18 +1363,19 @@ static int sur_fwd_bwd(mph_surrogate* s, const double4* pos, const double4* velf
     HIP_TRY(hipMemsetAsync(s->grad.p, 0, sizeof(float) * kSurNP, st));
     hipLaunchKernelGGL((k_sur_forward<true>), dim3(sur_grid(c)), dim3(256), 0, st, c->d_cases.p, c->n, s->plist.p,
                        s->nlist, s->toff.p, pos, velf, strain, c->pidx.p, s->pcnt.p, s->pnbr.p, s->wh, s->pk.p,
-                       s->w.p, sur_scales(s), s->apred.p, s->X.p, s->Z1.p, s->Z2.p, s->Z3.p);
+                       s->w.p, sur_scales(s), s->apred.p, s->X.p, s->Z1.p, s->Z2.p, s->Z3.p, s->Bs.p);
     hipLaunchKernelGGL(k_sur_loss, dim3((s->nlist + 255) / 256), dim3(256), 0, st, s->plist.p, s->nlist, s->apred.p,
                        s->target.p, (float)(1.0 / s->prm.a_scale), s->dmk.p, s->loss.p);
     const unsigned gb = (unsigned)((s->nslots + 255) / 256);
     hipLaunchKernelGGL((k_sur_backward<kSurOut, true>), dim3(gb), dim3(256), 0, st, s->nslots, s->towner.p, s->toff.p,
-                       s->plist.p, s->pcnt.p, s->dmk.p, s->wr.p, kOffW4, nullptr, s->Z3.p, s->dz3.p, s->dm.p);
+                       s->plist.p, s->pcnt.p, s->dmk.p, s->wr.p, kOffW4, nullptr, s->Z3.p, s->dz3.p, s->dm.p,
+                       s->Bs.p);
     hipLaunchKernelGGL((k_sur_backward<kSurHid, false>), dim3(gb), dim3(256), 0, st, s->nslots, s->towner.p,
                        s->toff.p, s->plist.p, s->pcnt.p, s->dmk.p, s->wr.p, kOffW3, s->dz3.p, s->Z2.p, s->dz2.p,
-                       nullptr);
+                       nullptr, nullptr);
     hipLaunchKernelGGL((k_sur_backward<kSurHid, false>), dim3(gb), dim3(256), 0, st, s->nslots, s->towner.p,
                        s->toff.p, s->plist.p, s->pcnt.p, s->dmk.p, s->wr.p, kOffW2, s->dz2.p, s->Z1.p, s->dz1.p,
-                       nullptr);
+                       nullptr, nullptr);
     const long nslices = s->nslots / 32;
     const int waves = 1024;
     const int per = (int)((nslices + waves - 1) / waves);

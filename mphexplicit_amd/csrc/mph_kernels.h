@@ -172,9 +172,11 @@ __device__ __forceinline__ int sub_bin(double t)
 __device__ __forceinline__ unsigned cell_key(const CaseConst& C, const int* __restrict__ rank,
                                              const double4& p, int type)
 {
-    const double tx = (p.x - C.dmin[0]) / C.cw[0];
-    const double ty = (p.y - C.dmin[1]) / C.cw[1];
-    const double tz = (p.z - C.dmin[2]) / C.cw[2];
+    // clamped so that any position, non-finite ones included, maps to a cell of the grid
+    // (fmax/fmin return the non-NaN operand); positions inside the domain are unchanged
+    const double tx = fmin(fmax((p.x - C.dmin[0]) / C.cw[0], -1.0e9), 1.0e9);
+    const double ty = fmin(fmax((p.y - C.dmin[1]) / C.cw[1], -1.0e9), 1.0e9);
+    const double tz = fmin(fmax((p.z - C.dmin[2]) / C.cw[2], -1.0e9), 1.0e9);
     int ix = ((int)floor(tx)) % C.cc[0];
     int iy = ((int)floor(ty)) % C.cc[1];
     int iz = ((int)floor(tz)) % C.cc[2];

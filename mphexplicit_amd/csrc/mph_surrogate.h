@@ -121,8 +121,9 @@ k_sur_pairs(const CaseConst* __restrict__ cases, const int* __restrict__ slot_ca
 __device__ __forceinline__ int sur_class(int t) { return t < 2 ? t : 2; }
 
 // Forward over the pairs of a list of particles.  Persistent wavefronts; one particle at a
-// time, 16 pairs per matrix tile.  TRAIN also stores the bf16 inputs X [P][32] and the FP32
-// pre-activations Z1..Z3 [P][64] of every pair slot P = 16 * toff[k] + e (e = pair number).
+// time, 16 pairs per matrix tile.  TRAIN also stores the bf16 inputs X [P][32], the FP32
+// pre-activations Z1..Z3 [P][64] and the message basis Bs [P][8] (e, u, u.e) of every pair
+// slot P = 16 * toff[k] + e (e = pair number).  apred.w holds a_scale * rho_0 / rho_i.
 // Inference (plist == nullptr) runs over all particles, skips walls and writes the
 // acceleration by original id.
 template <bool TRAIN>
@@ -132,7 +133,7 @@ k_sur_forward(const CaseConst* __restrict__ cases, int n, const int* __restrict_
               const double* __restrict__ strain, const int* __restrict__ pidx, const int* __restrict__ pcnt,
               const int* __restrict__ pnbr, int wh, const bf16x8* __restrict__ pk, const float* __restrict__ w,
               SurScales sc, float4* __restrict__ apred, __bf16* __restrict__ X, float* __restrict__ Z1,
-              float* __restrict__ Z2, float* __restrict__ Z3)
+              float* __restrict__ Z2, float* __restrict__ Z3, float* __restrict__ Bs)
 {
     const CaseConst& C = cases[0];
     const int lane = threadIdx.x & 63, p = lane & 15, g = lane >> 4;
@@ -174,7 +175,9 @@ k_sur_forward(const CaseConst* __restrict__ cases, int n, const int* __restrict_
             const int e = 16 * tt + p;
             const bool valid = e < c;
             const int j = valid ? pnbr[(size_t)i * wh + e] : i;
+            // features symmetric under i <-> j, and the antisymmetric basis of the message
             float f[16];
+            float ex, ey, ez, ux, uy, uz, ue;
             {
                 const double4 pj = pos[j];
                 double dxd, dyd, dzd;
@@ -183,22 +186,29 @@ k_sur_forward(const CaseConst* __restrict__ cases, int n, const int* __restrict_
                 const float r = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
                 const float ir = valid ? 1.0f / r : 0.0f;
                 const double4 vj = velf[j];
-                const float ux = (float)(vj.x - vi.x) * sc.inv_u, uy = (float)(vj.y - vi.y) * sc.inv_u;
-                const float uz = (float)(vj.z - vi.z) * sc.inv_u;
-                const float ex = dx * ir, ey = dy * ir, ez = dz * ir;
-                f[0] = ex; f[1] = ey; f[2] = ez;
-                f[3] = r * sc.inv_h;
-                f[4] = ux; f[5] = uy; f[6] = uz;
-                f[7] = fmaf(uz, ez, fmaf(uy, ey, ux * ex));
+                ux = (float)(vj.x - vi.x) * sc.inv_u;
+                uy = (float)(vj.y - vi.y) * sc.inv_u;
+                uz = (float)(vj.z - vi.z) * sc.inv_u;
+                ex = dx * ir; ey = dy * ir; ez = dz * ir;
+                ue = fmaf(uz, ez, fmaf(uy, ey, ux * ex));
+                const float s = r * sc.inv_h;
+                f[0] = s;
+                f[1] = s * s;
+                f[2] = ue;
+                f[3] = fmaf(uz, uz, fmaf(uy, uy, ux * ux));
                 const int ci = sur_class(ti), cj = sur_class(tag_of(pj.w) & 3);
-                for (int q = 0; q < 3; ++q) {
-                    f[8 + q] = ci == q ? 1.0f : 0.0f;
-                    f[11 + q] = cj == q ? 1.0f : 0.0f;
-                }
-                f[14] = di;
-                f[15] = (float)strain[j] * sc.inv_n0p * sc.dens_scale;
-                if (!valid)
+                const int lo = ci < cj ? ci : cj, hi = ci < cj ? cj : ci;
+                const int cp = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);   // unordered class pair
+                for (int q = 0; q < 6; ++q) f[4 + q] = cp == q ? 1.0f : 0.0f;
+                const float ej = (float)strain[j] * sc.inv_n0p * sc.dens_scale;
+                f[10] = di + ej;
+                f[11] = (di - ej) * (di - ej);
+                f[12] = di * ej;
+                f[13] = 0.0f; f[14] = 0.0f; f[15] = 0.0f;
+                if (!valid) {
                     for (int q = 0; q < 16; ++q) f[q] = 0.0f;
+                    ex = ey = ez = ux = uy = uz = ue = 0.0f;
+                }
             }
             bf16x8 b1;
             for (int q = 0; q < 8; ++q) {
@@ -239,8 +249,15 @@ k_sur_forward(const CaseConst* __restrict__ cases, int n, const int* __restrict_
             f32x4 o = bias4;
             o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A4[0], bh[0], o, 0, 0, 0);
             o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A4[1], bh[1], o, 0, 0, 0);
-            float mx = (g == 0 && valid) ? o[0] : 0.f, my = (g == 0 && valid) ? o[1] : 0.f;
-            float mz = (g == 0 && valid) ? o[2] : 0.f;
+            // m_ij = c0 e + c1 u + c2 (u.e) e: with symmetric coefficients m_ji = -m_ij
+            const float c0 = (g == 0 && valid) ? o[0] : 0.f, c1 = (g == 0 && valid) ? o[1] : 0.f;
+            const float c2 = (g == 0 && valid) ? o[2] : 0.f;
+            const float ce = fmaf(c2, ue, c0);
+            float mx = fmaf(c1, ux, ce * ex), my = fmaf(c1, uy, ce * ey), mz = fmaf(c1, uz, ce * ez);
+            if (TRAIN && g == 0) {
+                *reinterpret_cast<float4*>(Bs + P * 8) = make_float4(ex, ey, ez, ux);
+                *reinterpret_cast<float4*>(Bs + P * 8 + 4) = make_float4(uy, uz, ue, 0.f);
+            }
             for (int off = 8; off > 0; off >>= 1) {
                 mx += __shfl_xor(mx, off, 16);
                 my += __shfl_xor(my, off, 16);
@@ -249,8 +266,10 @@ k_sur_forward(const CaseConst* __restrict__ cases, int n, const int* __restrict_
             ax += mx; ay += my; az += mz;
         }
         if (lane == 0) {
-            const float4 a = make_float4(ax * sc.a_scale, ay * sc.a_scale, az * sc.a_scale, 0.f);
-            apred[plist ? k : pidx[i]] = a;
+            // messages are forces per unit volume of the reference density: a_i scales
+            // with rho_0 / rho_i
+            const float rr = (float)(C.mass[0] / C.mass[ti]) * sc.a_scale;
+            apred[plist ? k : pidx[i]] = make_float4(ax * rr, ay * rr, az * rr, rr);
         }
     }
 }
@@ -266,8 +285,9 @@ k_sur_loss(const int* __restrict__ plist, int nlist, const float4* __restrict__ 
         const float4 a = apred[k], t = target[k];
         const float dx = (a.x - t.x) * inv_a, dy = (a.y - t.y) * inv_a, dz = (a.z - t.z) * inv_a;
         l = (double)dx * dx + (double)dy * dy + (double)dz * dz;
-        // L = (1/N) sum |a - t|^2 / a_s^2 and a = a_s sum_j m_ij: dL/dm = 2 (a - t) / (N a_s)
-        const float c = 2.0f / (float)nlist;
+        // L = (1/N) sum |a - t|^2 / a_s^2 and a = s_i sum_j m_ij with s_i = apred.w:
+        // dL/dm = 2 (a - t) s_i / (N a_s^2)
+        const float c = 2.0f * a.w * inv_a / (float)nlist;
         dmk[k] = make_float4(c * dx, c * dy, c * dz, 0.f);
     }
     for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
@@ -276,14 +296,16 @@ k_sur_loss(const int* __restrict__ plist, int nlist, const float4* __restrict__ 
 
 // backward data path of one layer, one lane per pair slot (FP32):
 //   dz_in[a] = dsilu(Z[a]) * sum_b W[a][b] dz_out[b]
-// LAYER 4 takes dz_out = dL/dm of the slot's particle (0 for padding slots) and also stores
+// The first (output) layer takes dz_out = dL/dc, the message coefficients' gradient from
+// dL/dm of the slot's particle and the slot's basis (0 for padding slots), and also stores
 // it as the output gradient of the last layer ([P][4]).
 template <int NOUT, bool FIRST>
 __global__ void __launch_bounds__(256)
 k_sur_backward(long nslots, const int* __restrict__ tile_owner, const int* __restrict__ toff,
                const int* __restrict__ plist, const int* __restrict__ pcnt, const float4* __restrict__ dmk,
                const float* __restrict__ wr, int off_w, const float* __restrict__ dz_out,
-               const float* __restrict__ Z, float* __restrict__ dz_in, float* __restrict__ dm_store)
+               const float* __restrict__ Z, float* __restrict__ dz_in, float* __restrict__ dm_store,
+               const float* __restrict__ Bs)
 {
     __shared__ float W[kSurHid * NOUT];
     for (int q = threadIdx.x; q < kSurHid * NOUT; q += blockDim.x) W[q] = wr[off_w + q];
@@ -292,13 +314,18 @@ k_sur_backward(long nslots, const int* __restrict__ tile_owner, const int* __res
     if (P >= nslots) return;
     float d[NOUT];
     if (FIRST) {
+        // dL/dc of the message coefficients: m = c0 e + c1 u + c2 (u.e) e
         const int T = (int)(P >> 4), e0 = (int)(P & 15);
         const int k = tile_owner[T];
         const bool valid = 16 * (T - toff[k]) + e0 < pcnt[plist[k]];
         const float4 m = dmk[k];
-        d[0] = valid ? m.x : 0.f;
-        d[1] = valid ? m.y : 0.f;
-        d[2] = valid ? m.z : 0.f;
+        const float4 b0 = *reinterpret_cast<const float4*>(Bs + P * 8);
+        const float4 b1 = *reinterpret_cast<const float4*>(Bs + P * 8 + 4);
+        const float ge = fmaf(m.z, b0.z, fmaf(m.y, b0.y, m.x * b0.x));
+        const float gu = fmaf(m.z, b1.y, fmaf(m.y, b1.x, m.x * b0.w));
+        d[0] = valid ? ge : 0.f;
+        d[1] = valid ? gu : 0.f;
+        d[2] = valid ? ge * b1.z : 0.f;
         *reinterpret_cast<float4*>(dm_store + P * 4) = make_float4(d[0], d[1], d[2], 0.f);
     } else {
         for (int b = 0; b < NOUT; ++b) d[b] = dz_out[P * NOUT + b];

@@ -3,8 +3,9 @@
 Only tests/ may import this module; the product path (mphexplicit_amd) never does.
 
 It restates the network of the HIP kernels (mphexplicit_amd/csrc/mph_surrogate.h) in numpy
-FP64: the 17 pair features, three hidden SiLU layers of 64, a 3-wide message, the sum over
-the pairs of a particle, the loss and its analytic gradient.  With ``emulate=True`` it
+FP64: the 17 pair features (symmetric under i <-> j), three hidden SiLU layers of 64, three
+coefficients of the antisymmetric message m = c0 e + c1 u + c2 (u.e) e, the sum over the
+pairs of a particle, the loss and its analytic gradient.  With ``emulate=True`` it
 rounds where the GPU rounds (weights of W1..W4 and every matrix-core input to bfloat16,
 round-to-nearest-even); with ``emulate=False`` it is the exact FP64 function, which the
 central finite-difference check pins.
@@ -84,25 +85,45 @@ def pairs_within(fields, ptr, ind, lo, hi, maxr):
 
 
 def features(fields, pi, pj, lo, hi, maxr, n0p, u_scale, dens_scale):
-    """the 17 features of each pair (i, j): x_ij/r, r/h, u/u_s, u.e, class one-hots of i and j,
-    density deviations e^P/N0p of i and j, 1"""
+    """features of each pair (i, j), all symmetric under i <-> j, and the message basis.
+
+    X [P][17]: r/h, (r/h)^2, u.e, |u|^2, one-hot of the unordered class pair (00, 01, 02,
+    11, 12, 22; class = type 0, type 1, wall), e_i + e_j, (e_i - e_j)^2, e_i e_j (e = dens_scale
+    VolStrainP / N0p), three zeros and the constant 1.  Basis: e = x_ij / r, u = (v_j - v_i) /
+    u_scale and u.e, all antisymmetric except u.e, so m_ij = c0 e + c1 u + c2 (u.e) e is
+    antisymmetric."""
     x, v, n, t = fields["position"], fields["velocity"], fields["vol_strain_p"], fields["type"]
     d = periodic_diff(x[pi], x[pj], lo, hi)
     r = np.sqrt((d * d).sum(1))
     e = d / r[:, None]
     u = (v[pj] - v[pi]) / u_scale
+    ue = (u * e).sum(1)
     cls = np.minimum(t, 2)
+    a, b = np.minimum(cls[pi], cls[pj]), np.maximum(cls[pi], cls[pj])
+    cp = np.where(a == 0, b, np.where(a == 1, 2 + b, 5))
+    ei, ej = n[pi] / n0p * dens_scale, n[pj] / n0p * dens_scale
     X = np.zeros((len(pi), N_IN))
-    X[:, 0:3] = e
-    X[:, 3] = r / maxr
-    X[:, 4:7] = u
-    X[:, 7] = (u * e).sum(1)
-    X[np.arange(len(pi)), 8 + cls[pi]] = 1.0
-    X[np.arange(len(pi)), 11 + cls[pj]] = 1.0
-    X[:, 14] = n[pi] / n0p * dens_scale
-    X[:, 15] = n[pj] / n0p * dens_scale
+    X[:, 0] = r / maxr
+    X[:, 1] = (r / maxr) ** 2
+    X[:, 2] = ue
+    X[:, 3] = (u * u).sum(1)
+    X[np.arange(len(pi)), 4 + cp] = 1.0
+    X[:, 10] = ei + ej
+    X[:, 11] = (ei - ej) ** 2
+    X[:, 12] = ei * ej
     X[:, 16] = 1.0
-    return X
+    return X, dict(e=e, u=u, ue=ue)
+
+
+def mass_ratio(fields, density):
+    """rho_0 / rho_i of every particle (the messages are forces per unit volume)"""
+    rho = np.asarray(density, np.float64)
+    return rho[0] / rho[fields["type"]]
+
+
+def messages(C, basis):
+    """m = c0 e + c1 u + c2 (u.e) e from the network's outputs C [P][3]"""
+    return ((C[:, 0] + C[:, 2] * basis["ue"])[:, None] * basis["e"] + C[:, 1][:, None] * basis["u"])
 
 
 def forward(w, X, emulate=True):
@@ -121,40 +142,43 @@ def forward(w, X, emulate=True):
     return M, dict(X=Xr, Z1=Z1, H1=H1, Z2=Z2, H2=H2, Z3=Z3, H3=H3, W1=W1, W2=W2, W3=W3, W4=W4)
 
 
-def predict(w, X, pi, n, a_scale, emulate=True):
-    """a_i = a_scale * sum_j m_ij, [n][3] (zero for particles without pairs)"""
-    M, _ = forward(w, X, emulate)
-    a = np.zeros((n, 3))
-    np.add.at(a, pi, M)
-    return a_scale * a
+def predict(w, X, basis, pi, ratio, a_scale, emulate=True):
+    """a_i = a_scale rho_0/rho_i sum_j m_ij, [n][3] (zero for particles without pairs)"""
+    C, _ = forward(w, X, emulate)
+    a = np.zeros((len(ratio), 3))
+    np.add.at(a, pi, messages(C, basis))
+    return a_scale * ratio[:, None] * a
 
 
-def loss_and_grad(w, X, pi, train, target, a_scale, emulate=True):
+def loss_and_grad(w, X, basis, pi, ratio, train, target, a_scale, emulate=True):
     """L = mean over the training particles of |a - target|^2 / a_scale^2 and dL/dw.
 
-    train: original ids of the training particles; target [n][3].  With emulate, the inputs
-    of the weight-gradient products are rounded to bfloat16 as the matrix-core reduction does
-    (the FP32 data path is kept in FP64)."""
+    train: original ids of the training particles; target [n][3]; ratio = rho_0/rho_i.
+    With emulate, the inputs of the weight-gradient products are rounded to bfloat16 as the
+    matrix-core reduction does (the FP32 data path is kept in FP64)."""
     n = target.shape[0]
-    M, c = forward(w, X, emulate)
+    C, c = forward(w, X, emulate)
+    s = a_scale * ratio
     a = np.zeros((n, 3))
-    np.add.at(a, pi, M)
-    a *= a_scale
+    np.add.at(a, pi, messages(C, basis))
+    a *= s[:, None]
     mask = np.zeros(n, bool)
     mask[train] = True
     N = len(train)
     d = (a - target) / a_scale
     loss = float((d[mask] ** 2).sum() / N)
-    dA = np.where(mask[:, None], 2.0 * d / N, 0.0)          # dL/dm of each particle's pairs
-    dM = dA[pi]
+    dA = np.where(mask[:, None], 2.0 * d / (N * a_scale), 0.0)      # dL/da
+    g = (dA * s[:, None])[pi]                                        # dL/dm per pair
+    ge = (g * basis["e"]).sum(1)
+    dM = np.stack([ge, (g * basis["u"]).sum(1), ge * basis["ue"]], 1)  # dL/dc
     rw = bf16 if emulate else (lambda q: q)
     dz3 = (dM @ c["W4"].T) * dsilu(c["Z3"])
     dz2 = (dz3 @ c["W3"].T) * dsilu(c["Z2"])
     dz1 = (dz2 @ c["W2"].T) * dsilu(c["Z1"])
-    g = {
+    grad = {
         "W4": rw(c["H3"]).T @ rw(dM), "b4": rw(dM).sum(0),
         "W3": rw(c["H2"]).T @ rw(dz3), "b3": rw(dz3).sum(0),
         "W2": rw(c["H1"]).T @ rw(dz2), "b2": rw(dz2).sum(0),
         "W1": c["X"].T @ rw(dz1),
     }
-    return loss, flatten(g)
+    return loss, flatten(grad)

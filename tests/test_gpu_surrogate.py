@@ -40,8 +40,8 @@ def _oracle_inputs(deck, s):
     n0p, maxr = sc[1], sc[3]
     lo, hi = deck.domain_min, deck.domain_max
     pi, pj = S.pairs_within(f, ptr, ind, lo, hi, maxr)
-    X = S.features(f, pi, pj, lo, hi, maxr, n0p, SCALES["u_scale"], SCALES["dens_scale"])
-    return f, pi, X
+    X, basis = S.features(f, pi, pj, lo, hi, maxr, n0p, SCALES["u_scale"], SCALES["dens_scale"])
+    return f, pi, X, basis, S.mass_ratio(f, deck.data.density)
 
 
 def _state(mods, steps=6):
@@ -56,14 +56,29 @@ def test_forward_matches_oracle(mods):
     sur = mods.Surrogate(s, seed=7, **SCALES)
     w = sur.weights()
     a_gpu = sur.predict()
-    f, pi, X = _oracle_inputs(d, s)
-    a_orc = S.predict(w, X, pi, d.n, SCALES["a_scale"], emulate=True)
+    f, pi, X, basis, ratio = _oracle_inputs(d, s)
+    a_orc = S.predict(w, X, basis, pi, ratio, SCALES["a_scale"], emulate=True)
     fl = f["type"] < 2
     err = normwise(a_gpu[fl], a_orc[fl])
     print("surrogate forward normwise vs oracle %.2e, pairs %d" % (err, len(pi)), flush=True)
     # same bf16 roundings; differences come from FP32 vs FP64 arithmetic before a rounding
     assert err <= 2e-3
     assert np.all(a_gpu[~fl] == 0.0)
+
+
+def test_messages_conserve_momentum(mods):
+    # wall-free periodic droplet, one fluid: m_ji = -m_ij and symmetric pair sets make the
+    # predicted accelerations sum to zero up to FP32 rounding, for any weights
+    d = decks.small_droplet_2d()
+    s = mods.Solver.from_deck(d)
+    s.step(5)
+    sur = mods.Surrogate(s, seed=3, **SCALES)
+    rng = np.random.default_rng(0)
+    sur.set_weights(sur.weights() * 20.0 + rng.normal(size=sur.weights().shape).astype(np.float32) * 0.05)
+    a = sur.predict()
+    total, scale = np.abs(a.sum(0)).max(), np.abs(a).sum(0).max()
+    print("sum a %.3e of sum |a| %.3e" % (total, scale), flush=True)
+    assert total <= 1e-5 * scale
 
 
 def test_gradient_matches_oracle(mods):
@@ -73,9 +88,9 @@ def test_gradient_matches_oracle(mods):
     rng = np.random.default_rng(3)
     target = rng.normal(size=(d.n, 3)) * 5.0
     loss_g, g_gpu = sur.gradient(target)
-    f, pi, X = _oracle_inputs(d, s)
+    f, pi, X, basis, ratio = _oracle_inputs(d, s)
     train = np.nonzero(f["type"] < 2)[0]
-    loss_o, g_orc = S.loss_and_grad(w, X, pi, train, target, SCALES["a_scale"], emulate=True)
+    loss_o, g_orc = S.loss_and_grad(w, X, basis, pi, ratio, train, target, SCALES["a_scale"], emulate=True)
     print("loss gpu %.8e oracle %.8e" % (loss_g, loss_o), flush=True)
     assert abs(loss_g - loss_o) <= 1e-4 * loss_o
     o = 0
@@ -88,9 +103,9 @@ def test_gradient_matches_oracle(mods):
         o += m
 
 
-def _trained_weights(mods, deck, samples=40, iters=25):
+def _trained_weights(mods, deck, samples=300, iters=8, lr=1e-3):
     s = mods.Solver.from_deck(deck)
-    sur = mods.Surrogate(s, seed=5, lr=2e-3, **SCALES)
+    sur = mods.Surrogate(s, seed=5, lr=lr, **SCALES)
     losses = sur.train(samples, iters)
     w = sur.weights()
     sur.close()
@@ -99,7 +114,7 @@ def _trained_weights(mods, deck, samples=40, iters=25):
 
 
 def test_training_lowers_loss(mods):
-    w, losses = _trained_weights(mods, decks.small_dam_break_3d(), samples=20, iters=20)
+    w, losses = _trained_weights(mods, decks.small_dam_break_3d(), samples=20, iters=20, lr=2e-3)
     first, last = losses[:5].mean(), losses[-20:].mean()
     print("training loss first %.4e last %.4e" % (first, last), flush=True)
     assert np.all(np.isfinite(losses))
@@ -115,7 +130,7 @@ def _errors(a, b):
 
 def test_judge_and_gain(mods):
     deck = decks.small_dam_break_3d()
-    w, losses = _trained_weights(mods, deck, samples=60, iters=20)
+    w, losses = _trained_weights(mods, deck)
     print("training loss first %.4e last %.4e" % (losses[:5].mean(), losses[-20:].mean()), flush=True)
     names = ("velocity", "density_a", "type")
     W = 100                                 # plain steps before the comparison (flow developed)
@@ -140,9 +155,15 @@ def test_judge_and_gain(mods):
         sur = mods.Surrogate(s, seed=5, **SCALES)
         if weights is not None:
             sur.set_weights(weights)
-        sur.step(M)
-        assert abs(s.time() - ref.time()) <= 1e-12 * ref.time()
-        results[label] = _errors(s.fields(names=names), fr)
+        try:
+            sur.step(M)
+            results[label] = _errors(s.fields(names=names), fr)
+            assert abs(s.time() - ref.time()) <= 1e-12 * ref.time()
+        except mods.MphError as e:
+            # an unjudged run that compresses the fluid overflows a neighbour list: the
+            # library reports it instead of continuing with a wrong list
+            print(label, "diverged:", e, flush=True)
+            results[label] = (np.inf, np.inf)
         s2 = start()
         sur2 = mods.Surrogate(s2, seed=5, **SCALES)
         if weights is not None:
@@ -150,10 +171,32 @@ def test_judge_and_gain(mods):
         st = sur2.advance(blocks=3, k=10, tol=0.01)
         results[label + "_judged"] = st
         print(label, "judged", st, flush=True)
+        f2 = s2.fields(names=names)
+        assert np.all(np.isfinite(f2["velocity"]))       # rollbacks keep the state sane
     print("relative L2 (velocity, density) after %d dt from step %d: %s" % (
         2 * M, W, {k: v for k, v in results.items() if not k.endswith("judged")}), flush=True)
     tr, un = results["trained"], results["untrained"]
     # the trained network is closer to the plain run than the untrained one
     assert tr[0] < un[0] and tr[1] < un[1]
-    # the judge rejects a block whose one-step error is above the bound
-    assert results["untrained_judged"]["rollbacks"] >= 1
+    # the judge rejects every block of the untrained network and keeps blocks of the trained one
+    assert results["untrained_judged"]["rollbacks"] == 3
+    assert results["trained_judged"]["sur_steps"] > 0
+
+
+def test_judge_rolls_back_nan_weights(mods):
+    # corrupted weights: every block is rolled back and the state stays the plain path's
+    deck = decks.small_dam_break_3d()
+    s = mods.Solver.from_deck(deck)
+    s.step(20)
+    ref = mods.Solver.from_deck(deck)
+    ref.step(20 + 2 * 10 * 2)
+    sur = mods.Surrogate(s, seed=5, **SCALES)
+    w = sur.weights()
+    w[100:200] = np.nan
+    sur.set_weights(w)
+    st = sur.advance(blocks=2, k=10, tol=0.01)
+    assert st["rollbacks"] == 2 and st["sur_steps"] == 0
+    a, b = s.fields(names=("velocity", "position")), ref.fields(names=("velocity", "position"))
+    # the replayed blocks are plain steps: the result is the plain run's (same operations)
+    assert np.all(np.isfinite(a["velocity"]))
+    assert normwise(a["velocity"], b["velocity"]) <= 1e-9

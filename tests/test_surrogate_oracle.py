@@ -25,20 +25,25 @@ def _problem(seed=0, n=12, npairs=160):
     w = S.flatten(p)
     X = rng.normal(size=(npairs, 17))
     X[:, 16] = 1.0
+    e = rng.normal(size=(npairs, 3))
+    e /= np.linalg.norm(e, axis=1)[:, None]
+    u = rng.normal(size=(npairs, 3))
+    basis = dict(e=e, u=u, ue=(u * e).sum(1))
     pi = rng.integers(0, n, npairs)
+    ratio = rng.uniform(0.8, 1.25, n)
     target = rng.normal(size=(n, 3)) * 5.0
     train = np.arange(0, n, 2)
-    return w, X, pi, train, target
+    return w, X, basis, pi, ratio, train, target
 
 
-def _loss(w, X, pi, train, target, a_scale):
-    return S.loss_and_grad(w, X, pi, train, target, a_scale, emulate=False)[0]
+def _loss(w, X, basis, pi, ratio, train, target, a_scale):
+    return S.loss_and_grad(w, X, basis, pi, ratio, train, target, a_scale, emulate=False)[0]
 
 
 def test_gradient_matches_finite_differences():
-    w, X, pi, train, target = _problem()
+    w, X, basis, pi, ratio, train, target = _problem()
     a_scale = 3.0
-    loss, g = S.loss_and_grad(w, X, pi, train, target, a_scale, emulate=False)
+    loss, g = S.loss_and_grad(w, X, basis, pi, ratio, train, target, a_scale, emulate=False)
     rng = np.random.default_rng(1)
     # coordinates from every tensor, biases included
     idx, o = [], 0
@@ -52,16 +57,17 @@ def test_gradient_matches_finite_differences():
         wp, wm = w.copy(), w.copy()
         wp[k] += h
         wm[k] -= h
-        fd = (_loss(wp, X, pi, train, target, a_scale) - _loss(wm, X, pi, train, target, a_scale)) / (2 * h)
+        fd = (_loss(wp, X, basis, pi, ratio, train, target, a_scale) -
+              _loss(wm, X, basis, pi, ratio, train, target, a_scale)) / (2 * h)
         worst = max(worst, abs(fd - g[k]) / max(1e-3 * np.abs(g).max(), abs(g[k])))
     assert worst <= 1e-6, worst
 
 
 def test_planted_backward_error_is_caught(monkeypatch):
-    w, X, pi, train, target = _problem(seed=3)
+    w, X, basis, pi, ratio, train, target = _problem(seed=3)
     good = S.dsilu
     monkeypatch.setattr(S, "dsilu", lambda z: good(z) * (1.0 + 0.01 * np.sign(z)))   # a 1% slip
-    loss, g = S.loss_and_grad(w, X, pi, train, target, 2.0, emulate=False)
+    loss, g = S.loss_and_grad(w, X, basis, pi, ratio, train, target, 2.0, emulate=False)
     k = S.NPARAMS - 100       # a W4 weight: its gradient does not involve dsilu
     j = 100                   # a W1 weight: its gradient goes through all three dsilu
     h = 1e-6
@@ -69,7 +75,8 @@ def test_planted_backward_error_is_caught(monkeypatch):
         wp, wm = w.copy(), w.copy()
         wp[kk] += h
         wm[kk] -= h
-        fd = (_loss(wp, X, pi, train, target, 2.0) - _loss(wm, X, pi, train, target, 2.0)) / (2 * h)
+        fd = (_loss(wp, X, basis, pi, ratio, train, target, 2.0) -
+              _loss(wm, X, basis, pi, ratio, train, target, 2.0)) / (2 * h)
         rel = abs(fd - g[kk]) / max(abs(fd), 1e-12)
         assert (rel <= 1e-6) == should_match, (kk, rel)
 
@@ -79,18 +86,24 @@ def test_features_closed_form():
     f = {"position": np.array([[1.0, 5.0, 5.0], [3.0, 5.0, 5.0], [9.5, 5.0, 5.0], [5.0, 5.0, 5.0]]),
          "velocity": np.array([[0.0, 0.0, 0.0], [0.2, 0.0, 0.0], [0.0, 0.1, 0.0], [0.0, 0.0, 0.0]]),
          "vol_strain_p": np.array([0.0, 0.1, -0.1, 0.0]), "type": np.array([0, 1, 2, 0])}
-    pi, pj = np.array([0, 0]), np.array([1, 2])
-    X = S.features(f, pi, pj, lo, hi, maxr=3.5, n0p=1.0, u_scale=0.1, dens_scale=10.0)
-    # pair (0,1): along +x at r = 2
-    np.testing.assert_allclose(X[0, :4], [1.0, 0.0, 0.0, 2.0 / 3.5])
-    np.testing.assert_allclose(X[0, 4:8], [2.0, 0.0, 0.0, 2.0])
-    np.testing.assert_array_equal(X[0, 8:14], [1, 0, 0, 0, 1, 0])
-    np.testing.assert_allclose(X[0, 14:17], [0.0, 1.0, 1.0], atol=1e-12)
-    # pair (0,2): across the periodic face, x_j - x_i = -1.5
-    np.testing.assert_allclose(X[1, :4], [-1.0, 0.0, 0.0, 1.5 / 3.5])
-    np.testing.assert_allclose(X[1, 4:8], [0.0, 1.0, 0.0, 0.0], atol=1e-12)
-    np.testing.assert_array_equal(X[1, 8:14], [1, 0, 0, 0, 0, 1])
-    np.testing.assert_allclose(X[1, 15], -1.0)
+    pi, pj = np.array([0, 0, 1]), np.array([1, 2, 0])
+    X, B = S.features(f, pi, pj, lo, hi, maxr=3.5, n0p=1.0, u_scale=0.1, dens_scale=10.0)
+    # pair (0,1): along +x at r = 2, relative velocity (2, 0, 0) in units of u_scale
+    np.testing.assert_allclose(X[0, :4], [2.0 / 3.5, (2.0 / 3.5) ** 2, 2.0, 4.0])
+    np.testing.assert_array_equal(X[0, 4:10], [0, 1, 0, 0, 0, 0])        # classes {0, 1}
+    np.testing.assert_allclose(X[0, 10:17], [1.0, 1.0, 0.0, 0, 0, 0, 1.0], atol=1e-12)
+    np.testing.assert_allclose(B["e"][0], [1.0, 0.0, 0.0])
+    # pair (0,2): across the periodic face, x_j - x_i = -1.5; classes {0, wall}
+    np.testing.assert_allclose(X[1, :4], [1.5 / 3.5, (1.5 / 3.5) ** 2, 0.0, 1.0], atol=1e-12)
+    np.testing.assert_array_equal(X[1, 4:10], [0, 0, 1, 0, 0, 0])
+    np.testing.assert_allclose(B["e"][1], [-1.0, 0.0, 0.0])
+    # (1,0) is (0,1) seen from the other side: same features, opposite basis and message
+    np.testing.assert_allclose(X[2], X[0], atol=1e-12)
+    np.testing.assert_allclose(B["e"][2], -B["e"][0])
+    np.testing.assert_allclose(B["u"][2], -B["u"][0])
+    C = np.array([[0.3, -0.7, 0.2]] * 3)
+    m = S.messages(C, B)
+    np.testing.assert_allclose(m[2], -m[0], atol=1e-15)
 
 
 def test_bf16_rounding():
@@ -107,8 +120,8 @@ def test_bf16_rounding():
 
 
 def test_forward_emulation_close_to_exact():
-    w, X, pi, train, target = _problem(seed=5)
-    a = S.predict(w, X, pi, 12, 1.0, emulate=False)
-    b = S.predict(w, X, pi, 12, 1.0, emulate=True)
+    w, X, basis, pi, ratio, train, target = _problem(seed=5)
+    a = S.predict(w, X, basis, pi, ratio, 1.0, emulate=False)
+    b = S.predict(w, X, basis, pi, ratio, 1.0, emulate=True)
     # bfloat16 weights and activations: about 2^-8 relative per rounding
     assert np.max(np.abs(a - b)) / np.max(np.abs(a)) < 0.05
