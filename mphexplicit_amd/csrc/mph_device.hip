@@ -93,7 +93,7 @@ struct mph_ctx {
     DevBuf<double4> pos, pos_alt, vel, vel_alt, vnext, aux, force, snap;
     DevBuf<int> pidx, pidx_alt, nout, start;
     DevBuf<double> dens_a, vstrain, divp, wallc, vir;
-    DevBuf<unsigned> keys_a, keys_b, hist, ccount;
+    DevBuf<unsigned> keys_a, keys_b, hist, ccount, scan_sums;
     DevBuf<int> perm_a, perm_b, rank, cell_of_rank, cell_case, slot_case;
     DevBuf<int4> cl_info;
     DevBuf<int> cl_count, cl_nbr, cl_len;
@@ -354,6 +354,23 @@ static int launch_ok(const char* what)
     return e == hipSuccess ? MPH_OK : fail(MPH_ERR_DEVICE, "%s launch failed: %s", what, hipGetErrorString(e));
 }
 
+// exclusive scan of `total` counters in place: one workgroup for short arrays, the
+// three-phase multi-block scan for long ones (scan_sums holds the tile sums)
+static void launch_scan(mph_ctx* c, unsigned* data, int total)
+{
+    hipStream_t s = c->stream;
+    if (total <= 4 * kScanTile) {
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, data, total, c->any_flag.p);
+        return;
+    }
+    const int tiles = (total + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(tiles), dim3(kScanThreads), 0, s, data, total, c->scan_sums.p,
+                       c->any_flag.p);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, c->scan_sums.p, tiles, c->any_flag.p);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanThreads), 0, s, data, total, c->scan_sums.p,
+                       c->any_flag.p);
+}
+
 static int launch_rebuild(mph_ctx* c, int count_only)
 {
     const int nb = c->nblocks();
@@ -369,8 +386,7 @@ static int launch_rebuild(mph_ctx* c, int count_only)
         const int shift = ps * kRadixBits;
         hipLaunchKernelGGL(k_radix_hist, dim3(c->sort_blocks), dim3(kSortThreads), 0, s, kin, c->n,
                            shift, c->hist.p, c->any_flag.p);
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, c->hist.p,
-                           kRadix * c->sort_blocks, c->any_flag.p);
+        launch_scan(c, c->hist.p, kRadix * c->sort_blocks);
         hipLaunchKernelGGL(k_radix_scatter, dim3(c->sort_blocks), dim3(kSortThreads), 0, s, kin, vin,
                            kout, vout, c->n, shift, c->hist.p, c->any_flag.p);
     }
@@ -385,7 +401,7 @@ static int launch_rebuild(mph_ctx* c, int count_only)
     const int gcells = (c->ncells + 255) / 256;
     hipLaunchKernelGGL(k_cluster_count, dim3(gcells), dim3(256), 0, s, c->d_cases.p, c->cell_case.p,
                        c->ncells, c->start.p, c->K, c->ccount.p, c->any_flag.p);
-    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, c->ccount.p, c->ncells, c->any_flag.p);
+    launch_scan(c, c->ccount.p, c->ncells);
     hipLaunchKernelGGL(k_cluster_fill, dim3(gcells), dim3(256), 0, s, c->d_cases.p, c->cell_case.p,
                        c->ncells, c->start.p, c->cell_of_rank.p, c->K, c->ccount.p, c->cl_info.p,
                        c->cl_count.p, c->any_flag.p);
@@ -708,6 +724,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         (rc = c->perm_b.alloc(n)) || (rc = c->flags.alloc(ncases)) || (rc = c->any_flag.alloc(1)) ||
         (rc = c->rank.alloc(hrank.size())) || (rc = c->cell_of_rank.alloc(hcell.size())) ||
         (rc = c->cell_case.alloc(c->ncells)) || (rc = c->ccount.alloc(c->ncells)) ||
+        (rc = c->scan_sums.alloc((size_t)std::max(kRadix * c->sort_blocks, c->ncells) / kScanTile + 2)) ||
         (rc = c->cl_count.alloc(ncases)) || (rc = c->max_len.alloc(1)) ||
         (rc = c->rebuilds.alloc(ncases)) || (rc = c->pairs.alloc(kPairSlots * kPairStride)) ||
         (rc = c->pairs_scratch.alloc(kPairSlots * kPairStride)))

@@ -259,6 +259,68 @@ k_scan_exclusive(unsigned* __restrict__ data, int total, const int* __restrict__
     }
 }
 
+// Multi-block exclusive scan for long arrays (the cell counts of large batches): tile sums,
+// a single-workgroup scan of the tile sums (k_scan_exclusive), then a scan of each tile
+// from LDS with its tile's offset.  Tiles of kScanTile counters, coalesced loads.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_tile_sums(const unsigned* __restrict__ data, int total, unsigned* __restrict__ sums,
+                 const int* __restrict__ any_flag)
+{
+    if (any_flag && *any_flag == 0) return;
+    const int base = blockIdx.x * kScanTile;
+    unsigned s = 0;
+    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) s += base + k < total ? data[base + k] : 0u;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    __shared__ unsigned ws[kScanThreads / 64];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) t += ws[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_tiles(unsigned* __restrict__ data, int total, const unsigned* __restrict__ offsets,
+             const int* __restrict__ any_flag)
+{
+    if (any_flag && *any_flag == 0) return;
+    // padded so that the 16 consecutive counters of one thread fall in different banks
+    __shared__ unsigned tile[kScanTile + kScanTile / kScanItems];
+    __shared__ unsigned part[kScanThreads];
+    const int base = blockIdx.x * kScanTile;
+    auto at = [](int k) { return k + k / kScanItems; };
+    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads)
+        tile[at(k)] = base + k < total ? data[base + k] : 0u;
+    __syncthreads();
+    const int t = threadIdx.x;
+    unsigned s = 0;
+    for (int q = 0; q < kScanItems; ++q) s += tile[at(t * kScanItems + q)];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {   // inclusive scan of the thread sums
+        const unsigned v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    unsigned run = offsets[blockIdx.x] + part[t] - s;
+    for (int q = 0; q < kScanItems; ++q) {
+        const int k = at(t * kScanItems + q);
+        const unsigned v = tile[k];
+        tile[k] = run;
+        run += v;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads)
+        if (base + k < total) data[base + k] = tile[at(k)];
+}
+
 // stable scatter of one radix digit; ranks inside a tile come from 64-lane ballots
 __global__ void __launch_bounds__(kSortThreads)
 k_radix_scatter(const unsigned* __restrict__ kin, const int* __restrict__ vin,
