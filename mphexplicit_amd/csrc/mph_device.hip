@@ -1146,13 +1146,14 @@ static int sur_grid(const mph_ctx* c)
     return cus * 4;     // persistent: four 256-thread blocks per CU
 }
 
-// inference forward at (pos, velf, vstrain) over all fluid particles: s->apred by original id
+// inference forward at (pos, velf, vstrain) over all fluid particles, pair search fused:
+// s->apred by original id
 static int sur_infer(mph_surrogate* s, const double4* velf)
 {
     mph_ctx* c = s->c;
-    hipLaunchKernelGGL((k_sur_forward<false>), dim3(sur_grid(c)), dim3(256), 0, c->stream, c->d_cases.p, c->n,
-                       nullptr, 0, nullptr, c->pos.p, velf, c->vstrain.p, c->pidx.p, s->pcnt.p, s->pnbr.p, s->wh,
-                       s->pk.p, s->w.p, sur_scales(s), s->apred.p, nullptr, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_sur_infer, dim3(sur_grid(c)), dim3(256), 0, c->stream, c->d_cases.p, c->cl_info.p,
+                       c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->pos.p, velf, c->vstrain.p, c->pidx.p,
+                       s->pk.p, s->w.p, sur_scales(s), s->apred.p);
     return launch_ok("surrogate forward");
 }
 
@@ -1169,7 +1170,7 @@ static int sur_step(mph_surrogate* s)
 {
     mph_ctx* c = s->c;
     int rc = 0;
-    if ((rc = launch_move(c)) || (rc = launch_rebuild(c, 0)) || (rc = launch_sweep_a(c, 1)) || (rc = sur_pairs(s)) ||
+    if ((rc = launch_move(c)) || (rc = launch_rebuild(c, 0)) || (rc = launch_sweep_a(c, 1)) ||
         (rc = sur_infer(s, c->vel.p)) || (rc = sur_kick(s)) || (rc = launch_move(c)) || (rc = launch_rebuild(c, 0)) ||
         (rc = sur_kick(s)))
         return rc;
@@ -1299,7 +1300,7 @@ extern "C" int mph_sur_predict(mph_surrogate* s, double* accel)
     if (!s || !accel) return fail(MPH_ERR_ARG, "mph_sur_predict: bad argument");
     mph_ctx* c = s->c;
     int rc = 0;
-    if ((rc = sur_pairs(s)) || (rc = sur_infer(s, c->vnext.p))) return rc;
+    if ((rc = sur_infer(s, c->vnext.p))) return rc;
     std::vector<float4> h(c->n);
     HIP_TRY(hipMemcpyAsync(h.data(), s->apred.p, sizeof(float4) * c->n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1378,9 +1379,9 @@ static int sur_fwd_bwd(mph_surrogate* s, const double4* pos, const double4* velf
     hipStream_t st = c->stream;
     HIP_TRY(hipMemsetAsync(s->loss.p, 0, sizeof(double), st));
     HIP_TRY(hipMemsetAsync(s->grad.p, 0, sizeof(float) * kSurNP, st));
-    hipLaunchKernelGGL((k_sur_forward<true>), dim3(sur_grid(c)), dim3(256), 0, st, c->d_cases.p, c->n, s->plist.p,
-                       s->nlist, s->toff.p, pos, velf, strain, c->pidx.p, s->pcnt.p, s->pnbr.p, s->wh, s->pk.p,
-                       s->w.p, sur_scales(s), s->apred.p, s->X.p, s->Z1.p, s->Z2.p, s->Z3.p, s->Bs.p);
+    hipLaunchKernelGGL(k_sur_forward_train, dim3(sur_grid(c)), dim3(256), 0, st, c->d_cases.p, s->plist.p,
+                       s->nlist, s->toff.p, pos, velf, strain, s->pcnt.p, s->pnbr.p, s->wh, s->pk.p, s->w.p,
+                       sur_scales(s), s->apred.p, s->X.p, s->Z1.p, s->Z2.p, s->Z3.p, s->Bs.p);
     hipLaunchKernelGGL(k_sur_loss, dim3((s->nlist + 255) / 256), dim3(256), 0, st, s->plist.p, s->nlist, s->apred.p,
                        s->target.p, (float)(1.0 / s->prm.a_scale), s->dmk.p, s->loss.p);
     const unsigned gb = (unsigned)((s->nslots + 255) / 256);

@@ -120,52 +120,142 @@ k_sur_pairs(const CaseConst* __restrict__ cases, const int* __restrict__ slot_ca
 
 __device__ __forceinline__ int sur_class(int t) { return t < 2 ? t : 2; }
 
-// Forward over the pairs of a list of particles.  Persistent wavefronts; one particle at a
-// time, 16 pairs per matrix tile.  TRAIN also stores the bf16 inputs X [P][32], the FP32
-// pre-activations Z1..Z3 [P][64] and the message basis Bs [P][8] (e, u, u.e) of every pair
-// slot P = 16 * toff[k] + e (e = pair number).  apred.w holds a_scale * rho_0 / rho_i.
-// Inference (plist == nullptr) runs over all particles, skips walls and writes the
-// acceleration by original id.
+// Matrix-core operands of the network, loaded once per wavefront
+struct SurW {
+    bf16x8 A1[4], A2[4][2], A3[4][2], A4[2];
+    f32x4 bias2[4], bias3[4], bias4;
+};
+
+__device__ __forceinline__ void sur_load_weights(SurW& W, const bf16x8* __restrict__ pk, const float* __restrict__ w,
+                                                 int lane, int g)
+{
+    for (int t = 0; t < 4; ++t) {
+        W.A1[t] = pk[(kPkL1 + t) * 64 + lane];
+        for (int s = 0; s < 2; ++s) {
+            W.A2[t][s] = pk[(kPkL2 + 2 * t + s) * 64 + lane];
+            W.A3[t][s] = pk[(kPkL3 + 2 * t + s) * 64 + lane];
+        }
+    }
+    W.A4[0] = pk[(kPkL4 + 0) * 64 + lane];
+    W.A4[1] = pk[(kPkL4 + 1) * 64 + lane];
+    for (int t = 0; t < 4; ++t)
+        for (int q = 0; q < 4; ++q) {
+            W.bias2[t][q] = w[kOffB2 + 16 * t + 4 * g + q];
+            W.bias3[t][q] = w[kOffB3 + 16 * t + 4 * g + q];
+        }
+    for (int q = 0; q < 4; ++q) W.bias4[q] = (g == 0 && q < kSurOut) ? w[kOffB4 + q] : 0.0f;
+}
+
+// One tile of 16 pairs of particle i (lane l: pair l%16, lane group g = l/16): the
+// features, symmetric under i <-> j, the three layers on the matrix cores and the
+// antisymmetric message m = c0 e + c1 u + c2 (u.e) e.  Adds the tile's message sum to
+// (ax, ay, az) in every lane.  TRAIN stores X, Z1..Z3 and the basis at pair slot P.
 template <bool TRAIN>
+__device__ __forceinline__ void sur_tile(const SurW& W, const SurScales& sc, int g, bool valid, float dx, float dy,
+                                         float dz, int ci, int cj, float di, float dj, float ux, float uy, float uz,
+                                         size_t P, __bf16* __restrict__ X, float* __restrict__ Z1,
+                                         float* __restrict__ Z2, float* __restrict__ Z3, float* __restrict__ Bs,
+                                         float& ax, float& ay, float& az)
+{
+    float f[16];
+    float ex, ey, ez, ue;
+    {
+        const float r = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+        const float ir = valid ? 1.0f / r : 0.0f;
+        ex = dx * ir; ey = dy * ir; ez = dz * ir;
+        ue = fmaf(uz, ez, fmaf(uy, ey, ux * ex));
+        const float s = r * sc.inv_h;
+        f[0] = s;
+        f[1] = s * s;
+        f[2] = ue;
+        f[3] = fmaf(uz, uz, fmaf(uy, uy, ux * ux));
+        const int lo = ci < cj ? ci : cj, hi = ci < cj ? cj : ci;
+        const int cp = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);   // unordered class pair
+        for (int q = 0; q < 6; ++q) f[4 + q] = cp == q ? 1.0f : 0.0f;
+        f[10] = di + dj;
+        f[11] = (di - dj) * (di - dj);
+        f[12] = di * dj;
+        f[13] = 0.0f; f[14] = 0.0f; f[15] = 0.0f;
+        if (!valid) {
+            for (int q = 0; q < 16; ++q) f[q] = 0.0f;
+            ex = ey = ez = ux = uy = uz = ue = 0.0f;
+        }
+    }
+    bf16x8 b1;
+    for (int q = 0; q < 8; ++q) {
+        const float x = g == 0 ? f[q] : (g == 1 ? f[8 + q] : ((g == 2 && q == 0) ? 1.0f : 0.0f));
+        b1[q] = (__bf16)x;
+    }
+    if (TRAIN) *reinterpret_cast<bf16x8*>(X + P * 32 + 8 * g) = b1;
+    f32x4 z[4];
+    for (int t = 0; t < 4; ++t) {
+        z[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.A1[t], b1, z[t], 0, 0, 0);
+        if (TRAIN) *reinterpret_cast<f32x4*>(Z1 + P * 64 + 16 * t + 4 * g) = z[t];
+    }
+    bf16x8 bh[2];
+    for (int s = 0; s < 2; ++s)
+        for (int q = 0; q < 8; ++q) bh[s][q] = (__bf16)sur_silu(z[2 * s + (q >> 2)][q & 3]);
+    for (int t = 0; t < 4; ++t) {
+        z[t] = W.bias2[t];
+        z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.A2[t][0], bh[0], z[t], 0, 0, 0);
+        z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.A2[t][1], bh[1], z[t], 0, 0, 0);
+        if (TRAIN) *reinterpret_cast<f32x4*>(Z2 + P * 64 + 16 * t + 4 * g) = z[t];
+    }
+    for (int s = 0; s < 2; ++s)
+        for (int q = 0; q < 8; ++q) bh[s][q] = (__bf16)sur_silu(z[2 * s + (q >> 2)][q & 3]);
+    for (int t = 0; t < 4; ++t) {
+        z[t] = W.bias3[t];
+        z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.A3[t][0], bh[0], z[t], 0, 0, 0);
+        z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.A3[t][1], bh[1], z[t], 0, 0, 0);
+        if (TRAIN) *reinterpret_cast<f32x4*>(Z3 + P * 64 + 16 * t + 4 * g) = z[t];
+    }
+    for (int s = 0; s < 2; ++s)
+        for (int q = 0; q < 8; ++q) bh[s][q] = (__bf16)sur_silu(z[2 * s + (q >> 2)][q & 3]);
+    // output layer: rows 0..2 of lane group 0 are the coefficients of pair p
+    f32x4 o = W.bias4;
+    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.A4[0], bh[0], o, 0, 0, 0);
+    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.A4[1], bh[1], o, 0, 0, 0);
+    // m_ij = c0 e + c1 u + c2 (u.e) e: with symmetric coefficients m_ji = -m_ij
+    const float c0 = (g == 0 && valid) ? o[0] : 0.f, c1 = (g == 0 && valid) ? o[1] : 0.f;
+    const float c2 = (g == 0 && valid) ? o[2] : 0.f;
+    const float ce = fmaf(c2, ue, c0);
+    float mx = fmaf(c1, ux, ce * ex), my = fmaf(c1, uy, ce * ey), mz = fmaf(c1, uz, ce * ez);
+    if (TRAIN && g == 0) {
+        *reinterpret_cast<float4*>(Bs + P * 8) = make_float4(ex, ey, ez, ux);
+        *reinterpret_cast<float4*>(Bs + P * 8 + 4) = make_float4(uy, uz, ue, 0.f);
+    }
+    for (int off = 8; off > 0; off >>= 1) {
+        mx += __shfl_xor(mx, off, 16);
+        my += __shfl_xor(my, off, 16);
+        mz += __shfl_xor(mz, off, 16);
+    }
+    ax += mx; ay += my; az += mz;
+}
+
+// Training forward over a list of particles with precomputed pair lists (k_sur_pairs).
+// Persistent wavefronts, one particle at a time; stores the bf16 inputs X [P][32], the FP32
+// pre-activations Z1..Z3 [P][64] and the message basis Bs [P][8] of every pair slot
+// P = 16 * toff[k] + e (e = pair number).  apred[k] = a_i, with .w = a_scale * rho_0/rho_i.
 __global__ void __launch_bounds__(256)
-k_sur_forward(const CaseConst* __restrict__ cases, int n, const int* __restrict__ plist, int nlist,
-              const int* __restrict__ toff, const double4* __restrict__ pos, const double4* __restrict__ velf,
-              const double* __restrict__ strain, const int* __restrict__ pidx, const int* __restrict__ pcnt,
-              const int* __restrict__ pnbr, int wh, const bf16x8* __restrict__ pk, const float* __restrict__ w,
-              SurScales sc, float4* __restrict__ apred, __bf16* __restrict__ X, float* __restrict__ Z1,
-              float* __restrict__ Z2, float* __restrict__ Z3, float* __restrict__ Bs)
+k_sur_forward_train(const CaseConst* __restrict__ cases, const int* __restrict__ plist, int nlist,
+                    const int* __restrict__ toff, const double4* __restrict__ pos,
+                    const double4* __restrict__ velf, const double* __restrict__ strain,
+                    const int* __restrict__ pcnt, const int* __restrict__ pnbr, int wh,
+                    const bf16x8* __restrict__ pk, const float* __restrict__ w, SurScales sc,
+                    float4* __restrict__ apred, __bf16* __restrict__ X, float* __restrict__ Z1,
+                    float* __restrict__ Z2, float* __restrict__ Z3, float* __restrict__ Bs)
 {
     const CaseConst& C = cases[0];
     const int lane = threadIdx.x & 63, p = lane & 15, g = lane >> 4;
-    bf16x8 A1[4], A2[4][2], A3[4][2], A4[2];
-    for (int t = 0; t < 4; ++t) {
-        A1[t] = pk[(kPkL1 + t) * 64 + lane];
-        for (int s = 0; s < 2; ++s) {
-            A2[t][s] = pk[(kPkL2 + 2 * t + s) * 64 + lane];
-            A3[t][s] = pk[(kPkL3 + 2 * t + s) * 64 + lane];
-        }
-    }
-    A4[0] = pk[(kPkL4 + 0) * 64 + lane];
-    A4[1] = pk[(kPkL4 + 1) * 64 + lane];
-    f32x4 bias2[4], bias3[4], bias4;
-    for (int t = 0; t < 4; ++t)
-        for (int q = 0; q < 4; ++q) {
-            bias2[t][q] = w[kOffB2 + 16 * t + 4 * g + q];
-            bias3[t][q] = w[kOffB3 + 16 * t + 4 * g + q];
-        }
-    for (int q = 0; q < 4; ++q) bias4[q] = (g == 0 && q < kSurOut) ? w[kOffB4 + q] : 0.0f;
-
+    SurW W;
+    sur_load_weights(W, pk, w, lane, g);
     const int nw = gridDim.x * (blockDim.x >> 6);
     const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int total = plist ? nlist : n;
-    for (int k = wid; k < total; k += nw) {
-        const int i = plist ? plist[k] : k;
+    for (int k = wid; k < nlist; k += nw) {
+        const int i = plist[k];
         const double4 pi = pos[i];
         const int ti = tag_of(pi.w) & 3;
-        if (!plist && ti >= 2) {
-            if (lane == 0) apred[pidx[i]] = make_float4(0.f, 0.f, 0.f, 0.f);
-            continue;
-        }
         const double4 vi = velf[i];
         const float di = (float)strain[i] * sc.inv_n0p * sc.dens_scale;
         const int c = pcnt[i];
@@ -175,101 +265,122 @@ k_sur_forward(const CaseConst* __restrict__ cases, int n, const int* __restrict_
             const int e = 16 * tt + p;
             const bool valid = e < c;
             const int j = valid ? pnbr[(size_t)i * wh + e] : i;
-            // features symmetric under i <-> j, and the antisymmetric basis of the message
-            float f[16];
-            float ex, ey, ez, ux, uy, uz, ue;
-            {
-                const double4 pj = pos[j];
-                double dxd, dyd, dzd;
-                rel3(C, pi, pj, dxd, dyd, dzd);
-                const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
-                const float r = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-                const float ir = valid ? 1.0f / r : 0.0f;
-                const double4 vj = velf[j];
-                ux = (float)(vj.x - vi.x) * sc.inv_u;
-                uy = (float)(vj.y - vi.y) * sc.inv_u;
-                uz = (float)(vj.z - vi.z) * sc.inv_u;
-                ex = dx * ir; ey = dy * ir; ez = dz * ir;
-                ue = fmaf(uz, ez, fmaf(uy, ey, ux * ex));
-                const float s = r * sc.inv_h;
-                f[0] = s;
-                f[1] = s * s;
-                f[2] = ue;
-                f[3] = fmaf(uz, uz, fmaf(uy, uy, ux * ux));
-                const int ci = sur_class(ti), cj = sur_class(tag_of(pj.w) & 3);
-                const int lo = ci < cj ? ci : cj, hi = ci < cj ? cj : ci;
-                const int cp = lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);   // unordered class pair
-                for (int q = 0; q < 6; ++q) f[4 + q] = cp == q ? 1.0f : 0.0f;
-                const float ej = (float)strain[j] * sc.inv_n0p * sc.dens_scale;
-                f[10] = di + ej;
-                f[11] = (di - ej) * (di - ej);
-                f[12] = di * ej;
-                f[13] = 0.0f; f[14] = 0.0f; f[15] = 0.0f;
-                if (!valid) {
-                    for (int q = 0; q < 16; ++q) f[q] = 0.0f;
-                    ex = ey = ez = ux = uy = uz = ue = 0.0f;
-                }
-            }
-            bf16x8 b1;
-            for (int q = 0; q < 8; ++q) {
-                const float x = g == 0 ? f[q] : (g == 1 ? f[8 + q] : ((g == 2 && q == 0) ? 1.0f : 0.0f));
-                b1[q] = (__bf16)x;
-            }
-            const size_t P = TRAIN ? (size_t)16 * toff[k] + e : 0;
-            if (TRAIN) *reinterpret_cast<bf16x8*>(X + P * 32 + 8 * g) = b1;
-            // layer 1
-            f32x4 z[4];
-            for (int t = 0; t < 4; ++t) {
-                z[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-                z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[t], b1, z[t], 0, 0, 0);
-                if (TRAIN) *reinterpret_cast<f32x4*>(Z1 + P * 64 + 16 * t + 4 * g) = z[t];
-            }
-            bf16x8 bh[2];
-            for (int s = 0; s < 2; ++s)
-                for (int q = 0; q < 8; ++q) bh[s][q] = (__bf16)sur_silu(z[2 * s + (q >> 2)][q & 3]);
-            // layer 2
-            for (int t = 0; t < 4; ++t) {
-                z[t] = bias2[t];
-                z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[t][0], bh[0], z[t], 0, 0, 0);
-                z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[t][1], bh[1], z[t], 0, 0, 0);
-                if (TRAIN) *reinterpret_cast<f32x4*>(Z2 + P * 64 + 16 * t + 4 * g) = z[t];
-            }
-            for (int s = 0; s < 2; ++s)
-                for (int q = 0; q < 8; ++q) bh[s][q] = (__bf16)sur_silu(z[2 * s + (q >> 2)][q & 3]);
-            // layer 3
-            for (int t = 0; t < 4; ++t) {
-                z[t] = bias3[t];
-                z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A3[t][0], bh[0], z[t], 0, 0, 0);
-                z[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A3[t][1], bh[1], z[t], 0, 0, 0);
-                if (TRAIN) *reinterpret_cast<f32x4*>(Z3 + P * 64 + 16 * t + 4 * g) = z[t];
-            }
-            for (int s = 0; s < 2; ++s)
-                for (int q = 0; q < 8; ++q) bh[s][q] = (__bf16)sur_silu(z[2 * s + (q >> 2)][q & 3]);
-            // output layer: rows 0..2 of lane group 0 are the message of pair p
-            f32x4 o = bias4;
-            o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A4[0], bh[0], o, 0, 0, 0);
-            o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A4[1], bh[1], o, 0, 0, 0);
-            // m_ij = c0 e + c1 u + c2 (u.e) e: with symmetric coefficients m_ji = -m_ij
-            const float c0 = (g == 0 && valid) ? o[0] : 0.f, c1 = (g == 0 && valid) ? o[1] : 0.f;
-            const float c2 = (g == 0 && valid) ? o[2] : 0.f;
-            const float ce = fmaf(c2, ue, c0);
-            float mx = fmaf(c1, ux, ce * ex), my = fmaf(c1, uy, ce * ey), mz = fmaf(c1, uz, ce * ez);
-            if (TRAIN && g == 0) {
-                *reinterpret_cast<float4*>(Bs + P * 8) = make_float4(ex, ey, ez, ux);
-                *reinterpret_cast<float4*>(Bs + P * 8 + 4) = make_float4(uy, uz, ue, 0.f);
-            }
-            for (int off = 8; off > 0; off >>= 1) {
-                mx += __shfl_xor(mx, off, 16);
-                my += __shfl_xor(my, off, 16);
-                mz += __shfl_xor(mz, off, 16);
-            }
-            ax += mx; ay += my; az += mz;
+            const double4 pj = pos[j];
+            double dxd, dyd, dzd;
+            rel3(C, pi, pj, dxd, dyd, dzd);
+            const double4 vj = velf[j];
+            const float dj = (float)strain[j] * sc.inv_n0p * sc.dens_scale;
+            sur_tile<true>(W, sc, g, valid, (float)dxd, (float)dyd, (float)dzd, sur_class(ti),
+                           sur_class(tag_of(pj.w) & 3), di, dj, (float)(vj.x - vi.x) * sc.inv_u,
+                           (float)(vj.y - vi.y) * sc.inv_u, (float)(vj.z - vi.z) * sc.inv_u,
+                           (size_t)16 * toff[k] + e, X, Z1, Z2, Z3, Bs, ax, ay, az);
         }
         if (lane == 0) {
             // messages are forces per unit volume of the reference density: a_i scales
             // with rho_0 / rho_i
             const float rr = (float)(C.mass[0] / C.mass[ti]) * sc.a_scale;
-            apred[plist ? k : pidx[i]] = make_float4(ax * rr, ay * rr, az * rr, rr);
+            apred[k] = make_float4(ax * rr, ay * rr, az * rr, rr);
+        }
+    }
+}
+
+// Inference forward fused with the pair search: persistent wavefronts over the clusters;
+// for each fluid member the 64 lanes test 64 union-list entries at a time (exact periodic
+// difference, r <= h), ballot-compact the pairs with their FP32 offsets into this
+// wavefront's LDS queue, and every 16 queued pairs run one matrix tile.  Writes a_i by
+// original id (0 for walls).
+__global__ void __launch_bounds__(256)
+k_sur_infer(const CaseConst* __restrict__ cases, const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
+            const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
+            const double4* __restrict__ pos, const double4* __restrict__ velf, const double* __restrict__ strain,
+            const int* __restrict__ pidx, const bf16x8* __restrict__ pk, const float* __restrict__ w, SurScales sc,
+            float4* __restrict__ apred)
+{
+    constexpr int kQ = 80;                       // queue: 15 left over + 64 new at most
+    __shared__ int qj[4][kQ];
+    __shared__ float4 qd[4][kQ];                 // dx, dy, dz (FP32 offsets), class of j
+    const CaseConst& C = cases[0];
+    const int lane = threadIdx.x & 63, p = lane & 15, g = lane >> 4, wv = threadIdx.x >> 6;
+    SurW W;
+    sur_load_weights(W, pk, w, lane, g);
+    const int nw = gridDim.x * (blockDim.x >> 6);
+    const int wid = blockIdx.x * (blockDim.x >> 6) + wv;
+    const int ncl = cl_count[0];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int local = wid; local < ncl; local += nw) {
+        const int gsl = C.cl_base + local;
+        const int4 info = cl_info[gsl];
+        const int* list = cl_nbr + (size_t)gsl * width;
+        const int len = cl_len[gsl];
+        for (int m = 0; m < info.y; ++m) {
+            const int i = info.x + m;
+            const double4 pi = pos[i];
+            const int ti = tag_of(pi.w) & 3;
+            if (ti >= 2) {
+                if (lane == 0) apred[pidx[i]] = make_float4(0.f, 0.f, 0.f, 0.f);
+                continue;
+            }
+            const double4 vi = velf[i];
+            const float di = (float)strain[i] * sc.inv_n0p * sc.dens_scale;
+            const int ci = sur_class(ti);
+            float ax = 0.f, ay = 0.f, az = 0.f;
+            int count = 0;
+            auto run_tile = [&](int nvalid) {
+                __builtin_amdgcn_wave_barrier();
+                const bool valid = p < nvalid;
+                const int j = valid ? qj[wv][p] : i;
+                const float4 d = qd[wv][p];
+                const double4 vj = velf[j];
+                const float dj = (float)strain[j] * sc.inv_n0p * sc.dens_scale;
+                sur_tile<false>(W, sc, g, valid, d.x, d.y, d.z, ci, __float_as_int(d.w), di, dj,
+                                (float)(vj.x - vi.x) * sc.inv_u, (float)(vj.y - vi.y) * sc.inv_u,
+                                (float)(vj.z - vi.z) * sc.inv_u, 0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                ax, ay, az);
+            };
+            for (int e0 = 0; e0 < len; e0 += 64) {
+                const int e = e0 + lane;
+                const int j = e < len ? list[e] : -1;
+                bool ok = false;
+                double dxd = 0.0, dyd = 0.0, dzd = 0.0;
+                int cj = 0;
+                if (j >= 0 && j != i) {
+                    const double4 pj = pos[j];
+                    rel3(C, pi, pj, dxd, dyd, dzd);
+                    ok = dxd * dxd + dyd * dyd + dzd * dzd <= C.hmax2;
+                    cj = sur_class(tag_of(pj.w) & 3);
+                }
+                const unsigned long long mask = __ballot(ok);
+                if (ok) {
+                    const int at = count + (int)__popcll(mask & below);
+                    qj[wv][at] = j;
+                    qd[wv][at] = make_float4((float)dxd, (float)dyd, (float)dzd, __int_as_float(cj));
+                }
+                count += (int)__popcll(mask);
+                while (count >= 16) {
+                    run_tile(16);
+                    // move the rest of the queue down by one tile
+                    __builtin_amdgcn_wave_barrier();
+                    int mj = 0;
+                    float4 md = make_float4(0.f, 0.f, 0.f, 0.f);
+                    const bool mv = lane < count - 16;
+                    if (mv) {
+                        mj = qj[wv][lane + 16];
+                        md = qd[wv][lane + 16];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    if (mv) {
+                        qj[wv][lane] = mj;
+                        qd[wv][lane] = md;
+                    }
+                    count -= 16;
+                }
+            }
+            if (count > 0) run_tile(count);
+            if (lane == 0) {
+                const float rr = (float)(C.mass[0] / C.mass[ti]) * sc.a_scale;
+                apred[pidx[i]] = make_float4(ax * rr, ay * rr, az * rr, rr);
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
