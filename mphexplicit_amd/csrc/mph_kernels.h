@@ -76,14 +76,13 @@ __device__ __forceinline__ double rel_wrap(double xj, double xi, double hw, doub
 // ... and for a particle far enough from the periodic faces that no partner wraps
 __device__ __forceinline__ double rel_in(double xj, double xi, double hw) { return (xj - xi + hw) - hw; }
 
-// 1/sqrt(q2): hardware estimate and two Newton steps (about 1 ulp)
-__device__ __forceinline__ double rsqrt_nr(double q2)
+// 1/sqrt(q2) for the pair sweeps: the hardware estimate (measured on MI355X: at most
+// 5.2e-8 relative error) and one Newton step, at most 4.1e-15 relative (37 ulp) -- far
+// inside the FP64 field tolerance and three FP64 instructions per pair cheaper
+__device__ __forceinline__ double rsqrt_pair(double q2)
 {
-    double y = __builtin_amdgcn_rsq(q2);
-    const double h = 0.5 * q2;
-    double e = fma(-h * y, y, 0.5);
-    y = fma(y, e, y);
-    e = fma(-h * y, y, 0.5);
+    const double y = __builtin_amdgcn_rsq(q2);
+    const double e = fma(-(0.5 * q2) * y, y, 0.5);
     return fma(y, e, y);
 }
 
@@ -766,7 +765,7 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(q.p.z, pi.z, C.hwid[2]);
         const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
         acc.np += q2 <= C.hmax2;
-        const double ir = rsqrt_nr(q2);
+        const double ir = rsqrt_pair(q2);
         const double r = q2 * ir;
         const double sa = r * C.inv_ha;
         const double oma = fmax(1.0 - sa, 0.0);
@@ -901,7 +900,7 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
         const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(q.p.y, pi.y, C.hwid[1]);
         const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(q.p.z, pi.z, C.hwid[2]);
         const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        const double ir = rsqrt_nr(q2);
+        const double ir = rsqrt_pair(q2);
         const double r = q2 * ir;
         const int tj = tag_of(q.p.w) & 3;
         // pressure
