@@ -73,8 +73,10 @@ __device__ __forceinline__ double rel_wrap(double xj, double xi, double hw, doub
     return (a - s) - hw;
 }
 
-// ... and for a particle far enough from the periodic faces that no partner wraps
-__device__ __forceinline__ double rel_in(double xj, double xi, double hw) { return (xj - xi + hw) - hw; }
+// For a particle far enough from the periodic faces that no list partner wraps, the pair
+// sweeps use the plain difference x_j - x_i: the exact minimum image, which differs from
+// the reference's Mod form only by the rounding of the +-W/2 round trip.  Discrete
+// decisions (list cut-off, cell keys) keep the reference's form (rel3).
 
 // 1/sqrt(q2) for the pair sweeps: the hardware estimate (measured on MI355X: at most
 // 5.2e-8 relative error) and one Newton step, at most 4.1e-15 relative (37 ulp) -- far
@@ -760,9 +762,9 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         r.v = v4d{u.x, u.y, u.z, 0.0};
     };
     auto eval = [&](const Rec& q) {
-        const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(q.p.x, pi.x, C.hwid[0]);
-        const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(q.p.y, pi.y, C.hwid[1]);
-        const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(q.p.z, pi.z, C.hwid[2]);
+        const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : q.p.x - pi.x;
+        const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : q.p.y - pi.y;
+        const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : q.p.z - pi.z;
         const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
         acc.np += q2 <= C.hmax2;
         const double ir = rsqrt_pair(q2);
@@ -896,9 +898,9 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
         if (SURF) r.a = ld4(aux + j);
     };
     auto eval = [&](const Rec& q) {
-        const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : rel_in(q.p.x, pi.x, C.hwid[0]);
-        const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : rel_in(q.p.y, pi.y, C.hwid[1]);
-        const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : rel_in(q.p.z, pi.z, C.hwid[2]);
+        const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : q.p.x - pi.x;
+        const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : q.p.y - pi.y;
+        const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : q.p.z - pi.z;
         const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
         const double ir = rsqrt_pair(q2);
         const double r = q2 * ir;
