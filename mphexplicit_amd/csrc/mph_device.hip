@@ -331,6 +331,29 @@ static int derive_case(const mph_params& p, int begin, int end, int nfluid, int 
         }                                                                             \
     } while (0)
 
+#define LAUNCH_SKC(kern, SURF, K, COUNT, ...)                                         \
+    do {                                                                              \
+        if (COUNT) LAUNCH_SK_(kern, SURF, K, true, __VA_ARGS__);                      \
+        else LAUNCH_SK_(kern, SURF, K, false, __VA_ARGS__);                           \
+    } while (0)
+
+#define LAUNCH_SK_(kern, SURF, K, CB, ...)                                            \
+    do {                                                                              \
+        if (SURF) {                                                                   \
+            switch (K) {                                                              \
+            case 2: hipLaunchKernelGGL((kern<1, 2, CB>), __VA_ARGS__); break;         \
+            case 4: hipLaunchKernelGGL((kern<1, 4, CB>), __VA_ARGS__); break;         \
+            default: hipLaunchKernelGGL((kern<1, 8, CB>), __VA_ARGS__); break;        \
+            }                                                                         \
+        } else {                                                                      \
+            switch (K) {                                                              \
+            case 2: hipLaunchKernelGGL((kern<0, 2, CB>), __VA_ARGS__); break;         \
+            case 4: hipLaunchKernelGGL((kern<0, 4, CB>), __VA_ARGS__); break;         \
+            default: hipLaunchKernelGGL((kern<0, 8, CB>), __VA_ARGS__); break;        \
+            }                                                                         \
+        }                                                                             \
+    } while (0)
+
 #define LAUNCH_SK(kern, SURF, K, ...)                                                 \
     do {                                                                              \
         if (SURF) {                                                                   \
@@ -413,9 +436,9 @@ static int launch_rebuild(mph_ctx* c, int count_only)
     return launch_ok("rebuild");
 }
 
-// real: a step (advances wall centres, counts pairs); the autotuner launches with 0 so
-// that repeated launches leave the state unchanged
-static int launch_sweep_a(mph_ctx* c, int real)
+// real: a step (advances wall centres); the autotuner launches with 0 so that repeated
+// launches leave the state unchanged.  count: count the interacting pairs (timed steps)
+static int launch_sweep_a(mph_ctx* c, int real, bool count = false)
 {
     if (c->mixed) {
         LAUNCH_SK(k_sweep_a_mixed, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream,
@@ -424,7 +447,8 @@ static int launch_sweep_a(mph_ctx* c, int real)
                   real ? c->ncases : 0, real ? c->pairs.p : c->pairs_scratch.p);
         return launch_ok("sweep A");
     }
-    LAUNCH_SK(k_sweep_a, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p,
+    LAUNCH_SKC(k_sweep_a, c->any_surface, c->K, count && real, dim3(c->ncblocks()), dim3(256), 0, c->stream,
+               c->d_cases.p,
               c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p,
               c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p, c->wallc.p,
               real ? c->ncases : 0, real ? c->pairs.p : c->pairs_scratch.p);
@@ -837,7 +861,7 @@ extern "C" int mph_step_timed(mph_ctx* c, int nsteps, mph_timing* out)
         (void)hipEventRecord(e[1], c->stream);
         if (!rc) rc = launch_rebuild(c, 0);
         (void)hipEventRecord(e[2], c->stream);
-        if (!rc) rc = launch_sweep_a(c, 1);
+        if (!rc) rc = launch_sweep_a(c, 1, true);
         (void)hipEventRecord(e[3], c->stream);
         if (!rc) rc = launch_sweep_b(c, 1);
         (void)hipEventRecord(e[4], c->stream);

@@ -749,7 +749,7 @@ struct AccA {
     int np;
 };
 
-template <int SURF, int K, bool WRAP>
+template <int SURF, int K, bool WRAP, bool COUNT>
 __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, double vix, double viy, double viz,
                                               const double4* __restrict__ pos, const double4* vel,
@@ -766,7 +766,7 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : q.p.y - pi.y;
         const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : q.p.z - pi.z;
         const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
-        acc.np += q2 <= C.hmax2;
+        if (COUNT) acc.np += q2 <= C.hmax2;
         const double ir = rsqrt_pair(q2);
         const double r = q2 * ir;
         const double sa = r * C.inv_ha;
@@ -790,7 +790,9 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
 // without surface tension sweep A fits six wavefronts per SIMD (80 VGPRs, a few spilled
 // outside the pair loop), which measures faster than five; the other sweeps keep the
 // compiler's allocation
-template <int SURF, int K>
+// COUNT: also count the interacting pairs (r <= largest radius) into pair_count; only the
+// timed steps that report pair rates use it
+template <int SURF, int K, bool COUNT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 6, 8)))
 k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
@@ -829,9 +831,9 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     AccA acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
     // a wavefront takes the periodic path if any of its rows needs it (uniform branch)
     if (__ballot(!away(C, pi)) == 0ull)
-        sweep_a_pairs<SURF, K, false>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w], acc);
+        sweep_a_pairs<SURF, K, false, COUNT>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w], acc);
     else
-        sweep_a_pairs<SURF, K, true>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w], acc);
+        sweep_a_pairs<SURF, K, true, COUNT>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w], acc);
     acc.na = slot_sum<K>(acc.na);
     acc.ws = slot_sum<K>(acc.ws);
     acc.ds = slot_sum<K>(acc.ds);
@@ -840,7 +842,7 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
         acc.gy = slot_sum<K>(acc.gy);
         acc.gz = slot_sum<K>(acc.gz);
     }
-    int npair = slot_sum_i<K>(acc.np);
+    int npair = COUNT ? slot_sum_i<K>(acc.np) : 0;
     if (f.slot == 0 && f.valid) {
         const int i = f.i;
         if (count_out[i] == 0)                 // no list (main.cpp:1416): all sums are zero
@@ -861,10 +863,12 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     } else {
         npair = 0;
     }
-    for (int off = 32; off > 0; off >>= 1) npair += __shfl_xor(npair, off);
-    if ((threadIdx.x & 63) == 0 && npair) {
-        const int slot = (int)((blockIdx.x * 4u + (threadIdx.x >> 6)) & (kPairSlots - 1));
-        atomicAdd(pair_count + slot * kPairStride, (unsigned long long)npair);
+    if (COUNT) {
+        for (int off = 32; off > 0; off >>= 1) npair += __shfl_xor(npair, off);
+        if ((threadIdx.x & 63) == 0 && npair) {
+            const int slot = (int)((blockIdx.x * 4u + (threadIdx.x >> 6)) & (kPairSlots - 1));
+            atomicAdd(pair_count + slot * kPairStride, (unsigned long long)npair);
+        }
     }
 }
 
