@@ -81,6 +81,7 @@ struct mph_ctx {
     int ncases = 0, n = 0, ncells = 0;
     int nkeys = 0, sort_passes = 0, sort_blocks = 0, any_surface = 0;
     int mixed = 0;                 // MPH_MIXED: FP32 pair sweeps (all cases of a context)
+    int same_pv = 0;               // every case has equal pressure and viscosity radii
     int K = 4;                 // cluster size (rows per wavefront)
     int width = 0;             // reserved entries per cluster list
     int nslots = 0;            // cluster slots over all cases (capacity)
@@ -464,7 +465,8 @@ static int launch_sweep_b(mph_ctx* c, int real)
                   c->any_flag.p, c->rebuilds.p, real ? c->ncases : 0);
         return launch_ok("sweep B");
     }
-    LAUNCH_SK(k_sweep_b, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p,
+    LAUNCH_SKC(k_sweep_b, c->any_surface, c->K, c->same_pv, dim3(c->ncblocks()), dim3(256), 0, c->stream,
+               c->d_cases.p,
               c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p,
               c->pos.p, c->vel.p, c->aux.p, c->vnext_out(real), c->force_out(real), c->flags.p,
               c->any_flag.p, c->rebuilds.p, real ? c->ncases : 0);
@@ -679,6 +681,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     std::vector<int> hrank, hcell, hcell_case;
     std::vector<int2> logical;
     c->any_surface = 0;
+    c->same_pv = 1;
     c->mixed = prm[0].precision == MPH_MIXED;
     for (int k = 0; k < ncases; ++k) {
         const int nk = counts[k];
@@ -701,6 +704,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         key_base += 2 * ch.k.tcc;
         ch.time = prm[k].time;
         c->any_surface |= ch.k.surface;
+        if (ch.k.inv_hp != ch.k.inv_hv) c->same_pv = 0;
         for (int t = 0; t < kTypes; ++t)
             for (int d = 0; d < 3; ++d) hwall[((size_t)k * kTypes + t) * 3 + d] = prm[k].wall_center[t][d];
         for (int i = 0; i < nk; ++i) {

@@ -883,19 +883,20 @@ struct PairB {
 };
 
 struct AccB {
-    double px, py, pz;      // pressure
-    double vx, vy, vz;      // viscosity
+    double px, py, pz;      // pressure + viscosity, with their constant factors
+    double vx, vy, vz;      // (unused)
     double ax, ay, az;      // attractive pressure
     double sx, sy, sz;      // diffuse interface
 };
 
-template <int SURF, int K, bool WRAP>
+template <int SURF, int K, bool WRAP, bool SAMEPV>
 __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, const double4& vi, const double4& ai,
                                               const double4* __restrict__ pos, const double4* __restrict__ vel,
                                               const double4* __restrict__ aux, v2d* lds, int* ldsj, AccB& acc)
 {
     const double mu_i = tab[ti * 4 + ti];
+    const double kp = C.dcp * C.vol, kv = C.visc_c * (-C.dcv) * C.vol;
     auto fetch = [&](int j, Rec& r) {
         r.p = ld4(pos + j);
         r.v = ld4(vel + j);
@@ -909,18 +910,18 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
         const double ir = rsqrt_pair(q2);
         const double r = q2 * ir;
         const int tj = tag_of(q.p.w) & 3;
-        // pressure
-        const double cp = (vi.w + q.v.w) * fmax(fma(-r, C.inv_hp, 1.0), 0.0) * ir;
-        acc.px = fma(cp, dx, acc.px);
-        acc.py = fma(cp, dy, acc.py);
-        acc.pz = fma(cp, dz, acc.pz);
-        // viscosity with the harmonic-mean coefficient of the type pair
+        // pressure, and viscosity with the harmonic-mean coefficient of the type pair, both
+        // along x_ij: one coefficient with their constant factors (kp, kv)
+        const double wp = fmax(fma(-r, C.inv_hp, 1.0), 0.0);
+        const double wv = SAMEPV ? wp : fmax(fma(-r, C.inv_hv, 1.0), 0.0);
+        const double cp = (vi.w + q.v.w) * wp * ir;
         const double mu = C.mu_uniform ? mu_i : tab[ti * 4 + tj];
         const double ud = fma(q.v.z - vi.z, dz, fma(q.v.y - vi.y, dy, (q.v.x - vi.x) * dx));
-        const double cv = mu * ud * fmax(fma(-r, C.inv_hv, 1.0), 0.0) * (ir * ir * ir);
-        acc.vx = fma(cv, dx, acc.vx);
-        acc.vy = fma(cv, dy, acc.vy);
-        acc.vz = fma(cv, dz, acc.vz);
+        const double cv = mu * ud * wv * (ir * ir * ir);
+        const double c = fma(kp, cp, kv * cv);
+        acc.px = fma(c, dx, acc.px);
+        acc.py = fma(c, dy, acc.py);
+        acc.pz = fma(c, dz, acc.pz);
         if (SURF) {
             const double rij = C.ratio_one ? 1.0 : tab[16 + ti * 4 + tj];
             const double rji = C.ratio_one ? 1.0 : tab[16 + tj * 4 + ti];
@@ -945,7 +946,8 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
     walk_union_lds<K, SURF ? 3 : 2>(f, lds, ldsj, fetch, eval);
 }
 
-template <int SURF, int K>
+// SAMEPV: the pressure and viscosity kernels have the same radius (one weight per pair)
+template <int SURF, int K, bool SAMEPV>
 __global__ void __launch_bounds__(256)
 k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
@@ -985,11 +987,10 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     const double4 ai = SURF ? aux[f.i] : make_double4(0.0, 0.0, 0.0, 0.0);
     AccB acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (__ballot(!away(C, pi)) == 0ull)
-        sweep_b_pairs<SURF, K, false>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], ldsj[w], acc);
+        sweep_b_pairs<SURF, K, false, SAMEPV>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], ldsj[w], acc);
     else
-        sweep_b_pairs<SURF, K, true>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], ldsj[w], acc);
+        sweep_b_pairs<SURF, K, true, SAMEPV>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], ldsj[w], acc);
     acc.px = slot_sum<K>(acc.px); acc.py = slot_sum<K>(acc.py); acc.pz = slot_sum<K>(acc.pz);
-    acc.vx = slot_sum<K>(acc.vx); acc.vy = slot_sum<K>(acc.vy); acc.vz = slot_sum<K>(acc.vz);
     if (SURF) {
         acc.ax = slot_sum<K>(acc.ax); acc.ay = slot_sum<K>(acc.ay); acc.az = slot_sum<K>(acc.az);
         acc.sx = slot_sum<K>(acc.sx); acc.sy = slot_sum<K>(acc.sy); acc.sz = slot_sum<K>(acc.sz);
@@ -998,16 +999,13 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     const int i = f.i;
     if (count_out[i] == 0)                     // no list (main.cpp:1416): all sums are zero
         acc = AccB{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const double cp = C.dcp * C.vol;                      // pressure
-    const double cv = C.visc_c * (-C.dcv) * C.vol;        // viscosity
-    double fx = acc.px * cp, fy = acc.py * cp, fz = acc.pz * cp;
+    double fx = acc.px, fy = acc.py, fz = acc.pz;       // pressure + viscosity
     if (SURF) {
         const double ca = C.dca * C.vol;
         const double cs = C.adiff[ti] * C.s_scale;
         fx += acc.ax * ca; fy += acc.ay * ca; fz += acc.az * ca;
         fx += acc.sx * cs; fy += acc.sy * cs; fz += acc.sz * cs;
     }
-    fx += acc.vx * cv; fy += acc.vy * cv; fz += acc.vz * cv;
     double4 vn = make_double4(vi.x, vi.y, vi.z, 0.0);
     if (ti < 2) {
         const double ms = C.mass[ti];
