@@ -82,6 +82,7 @@ struct mph_ctx {
     int nkeys = 0, sort_passes = 0, sort_blocks = 0, any_surface = 0;
     int mixed = 0;                 // MPH_MIXED: FP32 pair sweeps (all cases of a context)
     int same_pv = 0;               // every case has equal pressure and viscosity radii
+    int same_ap = 0;               // every case has equal attractive and pressure radii
     int K = 4;                 // cluster size (rows per wavefront)
     int width = 0;             // reserved entries per cluster list
     int nslots = 0;            // cluster slots over all cases (capacity)
@@ -338,6 +339,34 @@ static int derive_case(const mph_params& p, int begin, int end, int nfluid, int 
         else LAUNCH_SK_(kern, SURF, K, false, __VA_ARGS__);                           \
     } while (0)
 
+#define LAUNCH_SKCC(kern, SURF, K, B1, B2, ...)                                      \
+    do {                                                                              \
+        if (B1) {                                                                     \
+            if (B2) LAUNCH_SK__(kern, SURF, K, true, true, __VA_ARGS__);              \
+            else LAUNCH_SK__(kern, SURF, K, true, false, __VA_ARGS__);                \
+        } else {                                                                      \
+            if (B2) LAUNCH_SK__(kern, SURF, K, false, true, __VA_ARGS__);             \
+            else LAUNCH_SK__(kern, SURF, K, false, false, __VA_ARGS__);               \
+        }                                                                             \
+    } while (0)
+
+#define LAUNCH_SK__(kern, SURF, K, CB, CB2, ...)                                      \
+    do {                                                                              \
+        if (SURF) {                                                                   \
+            switch (K) {                                                              \
+            case 2: hipLaunchKernelGGL((kern<1, 2, CB, CB2>), __VA_ARGS__); break;    \
+            case 4: hipLaunchKernelGGL((kern<1, 4, CB, CB2>), __VA_ARGS__); break;    \
+            default: hipLaunchKernelGGL((kern<1, 8, CB, CB2>), __VA_ARGS__); break;   \
+            }                                                                         \
+        } else {                                                                      \
+            switch (K) {                                                              \
+            case 2: hipLaunchKernelGGL((kern<0, 2, CB, CB2>), __VA_ARGS__); break;    \
+            case 4: hipLaunchKernelGGL((kern<0, 4, CB, CB2>), __VA_ARGS__); break;    \
+            default: hipLaunchKernelGGL((kern<0, 8, CB, CB2>), __VA_ARGS__); break;   \
+            }                                                                         \
+        }                                                                             \
+    } while (0)
+
 #define LAUNCH_SK_(kern, SURF, K, CB, ...)                                            \
     do {                                                                              \
         if (SURF) {                                                                   \
@@ -448,8 +477,8 @@ static int launch_sweep_a(mph_ctx* c, int real, bool count = false)
                   real ? c->ncases : 0, real ? c->pairs.p : c->pairs_scratch.p);
         return launch_ok("sweep A");
     }
-    LAUNCH_SKC(k_sweep_a, c->any_surface, c->K, count && real, dim3(c->ncblocks()), dim3(256), 0, c->stream,
-               c->d_cases.p,
+    LAUNCH_SKCC(k_sweep_a, c->any_surface, c->K, count && real, c->same_ap, dim3(c->ncblocks()), dim3(256), 0,
+                c->stream, c->d_cases.p,
               c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p,
               c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p, c->wallc.p,
               real ? c->ncases : 0, real ? c->pairs.p : c->pairs_scratch.p);
@@ -682,6 +711,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     std::vector<int2> logical;
     c->any_surface = 0;
     c->same_pv = 1;
+    c->same_ap = 1;
     c->mixed = prm[0].precision == MPH_MIXED;
     for (int k = 0; k < ncases; ++k) {
         const int nk = counts[k];
@@ -705,6 +735,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         ch.time = prm[k].time;
         c->any_surface |= ch.k.surface;
         if (ch.k.inv_hp != ch.k.inv_hv) c->same_pv = 0;
+        if (ch.k.inv_ha != ch.k.inv_hp) c->same_ap = 0;
         for (int t = 0; t < kTypes; ++t)
             for (int d = 0; d < 3; ++d) hwall[((size_t)k * kTypes + t) * 3 + d] = prm[k].wall_center[t][d];
         for (int i = 0; i < nk; ++i) {

@@ -749,7 +749,7 @@ struct AccA {
     int np;
 };
 
-template <int SURF, int K, bool WRAP, bool COUNT>
+template <int SURF, int K, bool WRAP, bool COUNT, bool SAMEAP>
 __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, double vix, double viy, double viz,
                                               const double4* __restrict__ pos, const double4* vel,
@@ -779,7 +779,7 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
             acc.gy = fma(dy, wa, acc.gy);
             acc.gz = fma(dz, wa, acc.gz);
         }
-        const double omp = fmax(fma(-r, C.inv_hp, 1.0), 0.0);
+        const double omp = SAMEAP ? oma : fmax(fma(-r, C.inv_hp, 1.0), 0.0);
         acc.ws = fma(omp, omp, acc.ws);
         const double ud = fma(q.v.z - viz, dz, fma(q.v.y - viy, dy, (q.v.x - vix) * dx));
         acc.ds = fma(ud, omp * ir, acc.ds);
@@ -791,8 +791,9 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
 // outside the pair loop), which measures faster than five; the other sweeps keep the
 // compiler's allocation
 // COUNT: also count the interacting pairs (r <= largest radius) into pair_count; only the
-// timed steps that report pair rates use it
-template <int SURF, int K, bool COUNT>
+// timed steps that report pair rates use it.  SAMEAP: the attractive and pressure kernels
+// have the same radius (one weight per pair)
+template <int SURF, int K, bool COUNT, bool SAMEAP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 6, 8)))
 k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
@@ -831,9 +832,11 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     AccA acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
     // a wavefront takes the periodic path if any of its rows needs it (uniform branch)
     if (__ballot(!away(C, pi)) == 0ull)
-        sweep_a_pairs<SURF, K, false, COUNT>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w], acc);
+        sweep_a_pairs<SURF, K, false, COUNT, SAMEAP>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w],
+                                                     acc);
     else
-        sweep_a_pairs<SURF, K, true, COUNT>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w], acc);
+        sweep_a_pairs<SURF, K, true, COUNT, SAMEAP>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w],
+                                                    acc);
     acc.na = slot_sum<K>(acc.na);
     acc.ws = slot_sum<K>(acc.ws);
     acc.ds = slot_sum<K>(acc.ds);
