@@ -88,6 +88,14 @@ __device__ __forceinline__ double rsqrt_pair(double q2)
     return fma(y, e, y);
 }
 
+// The pair sweeps evaluate every union entry without a branch.  Padding entries carry a
+// record far outside every support (kFar), and a row's own entry has q^2 == 0 exactly (the
+// same double on both sides); far_if_self maps that q^2 to one outside every support, so
+// both contribute exact zeros to every sum (0 * finite).  Two distinct particles at the
+// same position would make the reference divide by zero; here they do not interact.
+constexpr double kFar = 1.0e30;
+__device__ __forceinline__ double far_if_self(double q2) { return q2 > 0.0 ? q2 : 1.0e300; }
+
 // true if no neighbour-list partner of a particle at p can need a periodic image on any
 // axis: p is at least `guard` (list cut-off + Verlet skin) inside every face
 __device__ __forceinline__ bool away_from_faces(const CaseConst& C, const double4& p)
@@ -656,18 +664,21 @@ struct Rec {
 // slice, and then the K rows sweep the block in K steps of S slots, each row reading the
 // slot's record from LDS (K lanes read each address, a broadcast).  Every union entry is
 // fetched from global memory once per cluster instead of once per member, and the next
-// block's loads are in flight while the current block is evaluated.
+// block's loads are in flight while the current block is evaluated.  Entries are evaluated
+// without a branch: padding slots hold the far record and a row's own entry is caught by
+// far_if_self in the evaluators, so the loop over a block is straight-line code.
 template <int K, int NREC, class Fetch, class Eval>
-__device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restrict__ lds, int* __restrict__ ldsj,
-                                               Fetch fetch, Eval eval)
+__device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restrict__ lds, Fetch fetch, Eval eval)
 {
     // LDS slice: 2*NREC planes of 64 16-byte halves (x,y | z,w of each record), so the
     // 16 or 32 distinct entries one read touches are contiguous and free of bank conflicts
     constexpr int S = Frame<K>::S;
     const int lane = (int)(threadIdx.x & 63);
-    const int blocks = (f.len + 63) >> 6;
-    auto idx = [&](int blk) { const int e = (blk << 6) + lane; return e < f.len ? f.list[e] : -1; };
-    Rec r = {v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}};
+    const int len = __builtin_amdgcn_readfirstlane(f.len);      // one cluster per wavefront
+    const int blocks = (len + 63) >> 6;
+    auto idx = [&](int blk) { const int e = (blk << 6) + lane; return e < len ? f.list[e] : -1; };
+    const Rec far = {v4d{kFar, kFar, kFar, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}, v4d{0.0, 0.0, 0.0, 0.0}};
+    Rec r = far;
     // indices run two blocks ahead of the records, so a block's record loads never wait
     // on its index load
     int jr = idx(0), jn = idx(1);
@@ -686,18 +697,19 @@ __device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restric
             lds[4 * 64 + lane] = r.a.xy;
             lds[5 * 64 + lane] = r.a.zw;
         }
-        ldsj[lane] = jr;
         __builtin_amdgcn_wave_barrier();
         if (blk + 1 < blocks) {
             jr = jn;
             jn = idx(blk + 2);
-            if (jr >= 0) fetch(jr, r);
+            if (jr >= 0)
+                fetch(jr, r);
+            else
+                r = far;
         }
-        const int rest = f.len - (blk << 6);
+        const int rest = len - (blk << 6);
         const int nsteps = rest >= 64 ? K : (rest + S - 1) / S;
         for (int t = 0; t < nsteps; ++t) {
             const int e = t * S + f.slot;
-            const int j = ldsj[e];
             Rec q;
             q.p.xy = lds[0 * 64 + e];
             q.p.zw = lds[1 * 64 + e];
@@ -707,7 +719,7 @@ __device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restric
                 q.a.xy = lds[4 * 64 + e];
                 q.a.zw = lds[5 * 64 + e];
             }
-            if (j >= 0 && j != f.i && f.valid) eval(q);
+            eval(q);
         }
     }
 }
@@ -753,7 +765,7 @@ template <int SURF, int K, bool WRAP, bool COUNT, bool SAMEAP>
 __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, double vix, double viy, double viz,
                                               const double4* __restrict__ pos, const double4* vel,
-                                              v2d* lds, int* ldsj, AccA& acc)
+                                              v2d* lds, AccA& acc)
 {
     auto fetch = [&](int j, Rec& r) {
         r.p = ld4(pos + j);
@@ -765,7 +777,7 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : q.p.x - pi.x;
         const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : q.p.y - pi.y;
         const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : q.p.z - pi.z;
-        const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double q2 = far_if_self(fma(dz, dz, fma(dy, dy, dx * dx)));
         if (COUNT) acc.np += q2 <= C.hmax2;
         const double ir = rsqrt_pair(q2);
         const double r = q2 * ir;
@@ -784,17 +796,18 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         const double ud = fma(q.v.z - viz, dz, fma(q.v.y - viy, dy, (q.v.x - vix) * dx));
         acc.ds = fma(ud, omp * ir, acc.ds);
     };
-    walk_union_lds<K, 2>(f, lds, ldsj, fetch, eval);
+    walk_union_lds<K, 2>(f, lds, fetch, eval);
 }
 
-// without surface tension sweep A fits six wavefronts per SIMD (80 VGPRs, a few spilled
-// outside the pair loop), which measures faster than five; the other sweeps keep the
-// compiler's allocation
+// Without surface tension both sweeps are held to five wavefronts per SIMD (at most 96
+// VGPRs; a few bytes are spilled outside the pair loop).  Left alone the compiler takes 100
+// for the straight-line pair loop, which leaves four wavefronts and measures slower.  Six
+// (80 VGPRs) spills inside the block loop.
 // COUNT: also count the interacting pairs (r <= largest radius) into pair_count; only the
 // timed steps that report pair rates use it.  SAMEAP: the attractive and pressure kernels
 // have the same radius (one weight per pair)
 template <int SURF, int K, bool COUNT, bool SAMEAP>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 6, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 5, 8)))
 k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
           const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
@@ -817,7 +830,6 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     __shared__ double tab[32];                // [0,16): mu_ij, [16,32): InteractionRatio
     constexpr int S = Frame<K>::S;
     __shared__ v2d lds[4][4 * 64];        // per wavefront: {pos, vel} halves x 64 entries
-    __shared__ int ldsj[4][64];
     if (threadIdx.x < 16) {
         tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
         tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
@@ -832,10 +844,10 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     AccA acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
     // a wavefront takes the periodic path if any of its rows needs it (uniform branch)
     if (__ballot(!away(C, pi)) == 0ull)
-        sweep_a_pairs<SURF, K, false, COUNT, SAMEAP>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w],
+        sweep_a_pairs<SURF, K, false, COUNT, SAMEAP>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w],
                                                      acc);
     else
-        sweep_a_pairs<SURF, K, true, COUNT, SAMEAP>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], ldsj[w],
+        sweep_a_pairs<SURF, K, true, COUNT, SAMEAP>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w],
                                                     acc);
     acc.na = slot_sum<K>(acc.na);
     acc.ws = slot_sum<K>(acc.ws);
@@ -896,7 +908,7 @@ template <int SURF, int K, bool WRAP, bool SAMEPV>
 __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, const double4& vi, const double4& ai,
                                               const double4* __restrict__ pos, const double4* __restrict__ vel,
-                                              const double4* __restrict__ aux, v2d* lds, int* ldsj, AccB& acc)
+                                              const double4* __restrict__ aux, v2d* lds, AccB& acc)
 {
     const double mu_i = tab[ti * 4 + ti];
     const double kp = C.dcp * C.vol, kv = C.visc_c * (-C.dcv) * C.vol;
@@ -909,7 +921,7 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
         const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : q.p.x - pi.x;
         const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : q.p.y - pi.y;
         const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : q.p.z - pi.z;
-        const double q2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double q2 = far_if_self(fma(dz, dz, fma(dy, dy, dx * dx)));
         const double ir = rsqrt_pair(q2);
         const double r = q2 * ir;
         const int tj = tag_of(q.p.w) & 3;
@@ -946,12 +958,13 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
             acc.sz = fma(-om, fma(qd, dz, w * qz), acc.sz);
         }
     };
-    walk_union_lds<K, SURF ? 3 : 2>(f, lds, ldsj, fetch, eval);
+    walk_union_lds<K, SURF ? 3 : 2>(f, lds, fetch, eval);
 }
 
-// SAMEPV: the pressure and viscosity kernels have the same radius (one weight per pair)
+// SAMEPV: the pressure and viscosity kernels have the same radius (one weight per pair).
+// Five wavefronts per SIMD without surface tension, as sweep A.
 template <int SURF, int K, bool SAMEPV>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 5, 8)))
 k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
           const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
@@ -975,7 +988,6 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     constexpr int S = Frame<K>::S;
     constexpr int NREC = SURF ? 3 : 2;
     __shared__ v2d lds[4][2 * NREC * 64]; // per wavefront: record halves x 64 entries
-    __shared__ int ldsj[4][64];
     if (threadIdx.x < 16) {
         tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
         tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
@@ -990,9 +1002,9 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     const double4 ai = SURF ? aux[f.i] : make_double4(0.0, 0.0, 0.0, 0.0);
     AccB acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (__ballot(!away(C, pi)) == 0ull)
-        sweep_b_pairs<SURF, K, false, SAMEPV>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], ldsj[w], acc);
+        sweep_b_pairs<SURF, K, false, SAMEPV>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], acc);
     else
-        sweep_b_pairs<SURF, K, true, SAMEPV>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], ldsj[w], acc);
+        sweep_b_pairs<SURF, K, true, SAMEPV>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], acc);
     acc.px = slot_sum<K>(acc.px); acc.py = slot_sum<K>(acc.py); acc.pz = slot_sum<K>(acc.pz);
     if (SURF) {
         acc.ax = slot_sum<K>(acc.ax); acc.ay = slot_sum<K>(acc.ay); acc.az = slot_sum<K>(acc.az);

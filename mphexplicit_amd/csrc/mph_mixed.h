@@ -37,16 +37,23 @@ __device__ __forceinline__ float offset_f(double x, double xf, double hw, double
     return (float)(WRAP ? rel_wrap(x, xf, hw, w) : x - xf);
 }
 
+// FP32 counterparts of kFar and far_if_self: squares of the far offset stay finite in FP32
+constexpr float kFarF = 1.0e18f;
+__device__ __forceinline__ float far_if_self_f(float q2) { return q2 > 0.f ? q2 : 1.0e36f; }
+
+
 template <int K, int NREC, class Fetch, class Eval>
-__device__ __forceinline__ void walk_union_lds_f(const Frame<K>& f, v4f* __restrict__ lds, int* __restrict__ ldsj,
-                                                 Fetch fetch, Eval eval)
+__device__ __forceinline__ void walk_union_lds_f(const Frame<K>& f, v4f* __restrict__ lds, Fetch fetch, Eval eval)
 {
-    // the FP32 counterpart of walk_union_lds: NREC planes of 64 float4 records
+    // the FP32 counterpart of walk_union_lds (branch-free entries, the far record in
+    // padding slots): NREC planes of 64 float4 records
     constexpr int S = Frame<K>::S;
     const int lane = (int)(threadIdx.x & 63);
-    const int blocks = (f.len + 63) >> 6;
-    auto idx = [&](int blk) { const int e = (blk << 6) + lane; return e < f.len ? f.list[e] : -1; };
-    RecF r = {v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
+    const int len = __builtin_amdgcn_readfirstlane(f.len);
+    const int blocks = (len + 63) >> 6;
+    auto idx = [&](int blk) { const int e = (blk << 6) + lane; return e < len ? f.list[e] : -1; };
+    const RecF far = {v4f{kFarF, kFarF, kFarF, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
+    RecF r = far;
     int jr = idx(0), jn = idx(1);
     if (jr >= 0) fetch(jr, r);
     for (int blk = 0; blk < blocks; ++blk) {
@@ -56,23 +63,24 @@ __device__ __forceinline__ void walk_union_lds_f(const Frame<K>& f, v4f* __restr
         lds[0 * 64 + lane] = r.p;
         lds[1 * 64 + lane] = r.v;
         if (NREC > 2) lds[2 * 64 + lane] = r.a;
-        ldsj[lane] = jr;
         __builtin_amdgcn_wave_barrier();
         if (blk + 1 < blocks) {
             jr = jn;
             jn = idx(blk + 2);
-            if (jr >= 0) fetch(jr, r);
+            if (jr >= 0)
+                fetch(jr, r);
+            else
+                r = far;
         }
-        const int rest = f.len - (blk << 6);
+        const int rest = len - (blk << 6);
         const int nsteps = rest >= 64 ? K : (rest + S - 1) / S;
         for (int t = 0; t < nsteps; ++t) {
             const int e = t * S + f.slot;
-            const int j = ldsj[e];
             RecF q;
             q.p = lds[0 * 64 + e];
             q.v = lds[1 * 64 + e];
             if (NREC > 2) q.a = lds[2 * 64 + e];
-            if (j >= 0 && j != f.i && f.valid) eval(q);
+            eval(q);
         }
     }
 }
@@ -86,7 +94,7 @@ __device__ __forceinline__ void sweep_a_pairs_f(const CaseConst& C, const float*
                                                 const double4& pf, float ox, float oy, float oz,
                                                 float vix, float viy, float viz,
                                                 const double4* __restrict__ pos, const double4* vel,
-                                                v4f* lds, int* ldsj, AccA& acc, float& gxf, float& gyf,
+                                                v4f* lds, AccA& acc, float& gxf, float& gyf,
                                                 float& gzf)
 {
     const float hwx = (float)C.hwid[0], hwy = (float)C.hwid[1], hwz = (float)C.hwid[2];
@@ -106,7 +114,7 @@ __device__ __forceinline__ void sweep_a_pairs_f(const CaseConst& C, const float*
             dy = fold_f(dy, hwy, wy);
             dz = fold_f(dz, hwz, wz);
         }
-        const float q2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float q2 = far_if_self_f(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
         acc.np += q2 <= hmax2;
         const float ir = __builtin_amdgcn_rsqf(q2);
         const float r = q2 * ir;
@@ -125,11 +133,14 @@ __device__ __forceinline__ void sweep_a_pairs_f(const CaseConst& C, const float*
         const float ud = fmaf(q.v.z - viz, dz, fmaf(q.v.y - viy, dy, (q.v.x - vix) * dx));
         acc.ds += (double)(ud * (omp * ir));
     };
-    walk_union_lds_f<K, 2>(f, lds, ldsj, fetch, eval);
+    walk_union_lds_f<K, 2>(f, lds, fetch, eval);
 }
 
+// Without surface tension the mixed sweeps are held to six wavefronts per SIMD (80 VGPRs,
+// some spilled outside the pair loop): measured faster than five (96 VGPRs) and than the
+// compiler's choice (99 for sweep B, four wavefronts).
 template <int SURF, int K>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 6, 8)))
 k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
                 const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
                 const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
@@ -150,7 +161,6 @@ k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
     const CaseConst& C = cases[m.x];
     __shared__ float tabf[32];                 // [16,32): InteractionRatio
     __shared__ v4f lds[4][2 * 64];             // per wavefront: {offset, vel} x 64 entries
-    __shared__ int ldsj[4][64];
     if (threadIdx.x < 16) {
         tabf[threadIdx.x] = (float)C.muij[threadIdx.x >> 2][threadIdx.x & 3];
         tabf[16 + threadIdx.x] = (float)C.ir[threadIdx.x >> 2][threadIdx.x & 3];
@@ -169,13 +179,13 @@ k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
         const float ox = offset_f<false>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
         const float oy = offset_f<false>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
         const float oz = offset_f<false>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
-        sweep_a_pairs_f<SURF, K, false>(C, tabf, f, ti, pf, ox, oy, oz, vix, viy, viz, pos, vel, lds[w], ldsj[w],
+        sweep_a_pairs_f<SURF, K, false>(C, tabf, f, ti, pf, ox, oy, oz, vix, viy, viz, pos, vel, lds[w],
                                         acc, gxf, gyf, gzf);
     } else {
         const float ox = offset_f<true>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
         const float oy = offset_f<true>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
         const float oz = offset_f<true>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
-        sweep_a_pairs_f<SURF, K, true>(C, tabf, f, ti, pf, ox, oy, oz, vix, viy, viz, pos, vel, lds[w], ldsj[w],
+        sweep_a_pairs_f<SURF, K, true>(C, tabf, f, ti, pf, ox, oy, oz, vix, viy, viz, pos, vel, lds[w],
                                        acc, gxf, gyf, gzf);
     }
     acc.na = slot_sum<K>(acc.na);
@@ -227,7 +237,7 @@ __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float*
                                                 const double4& pf, float ox, float oy, float oz,
                                                 const v4f& vi, const v4f& ai,
                                                 const double4* __restrict__ pos, const double4* __restrict__ vel,
-                                                const double4* __restrict__ aux, v4f* lds, int* ldsj, AccBF& acc)
+                                                const double4* __restrict__ aux, v4f* lds, AccBF& acc)
 {
     const float hwx = (float)C.hwid[0], hwy = (float)C.hwid[1], hwz = (float)C.hwid[2];
     const float wx = (float)C.dwid[0], wy = (float)C.dwid[1], wz = (float)C.dwid[2];
@@ -252,7 +262,7 @@ __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float*
             dy = fold_f(dy, hwy, wy);
             dz = fold_f(dz, hwz, wz);
         }
-        const float q2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const float q2 = far_if_self_f(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
         const float ir = __builtin_amdgcn_rsqf(q2);
         const float r = q2 * ir;
         const int tj = __float_as_int(q.p.w);
@@ -286,7 +296,7 @@ __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float*
             acc.sz = fmaf(-om, fmaf(qd, dz, wg * qz), acc.sz);
         }
     };
-    walk_union_lds_f<K, SURF ? 3 : 2>(f, lds, ldsj, fetch, eval);
+    walk_union_lds_f<K, SURF ? 3 : 2>(f, lds, fetch, eval);
 }
 
 template <int K>
@@ -297,7 +307,7 @@ __device__ __forceinline__ float slot_sum_f(float v)
 }
 
 template <int SURF, int K>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 6, 8)))
 k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
                 const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
                 const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
@@ -320,7 +330,6 @@ k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
     __shared__ float tabf[32];
     constexpr int NREC = SURF ? 3 : 2;
     __shared__ v4f lds[4][NREC * 64];          // per wavefront: records x 64 entries
-    __shared__ int ldsj[4][64];
     if (threadIdx.x < 16) {
         tabf[threadIdx.x] = (float)C.muij[threadIdx.x >> 2][threadIdx.x & 3];
         tabf[16 + threadIdx.x] = (float)C.ir[threadIdx.x >> 2][threadIdx.x & 3];
@@ -341,13 +350,13 @@ k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
         const float ox = offset_f<false>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
         const float oy = offset_f<false>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
         const float oz = offset_f<false>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
-        sweep_b_pairs_f<SURF, K, false>(C, tabf, f, ti, pf, ox, oy, oz, vif, aif, pos, vel, aux, lds[w], ldsj[w],
+        sweep_b_pairs_f<SURF, K, false>(C, tabf, f, ti, pf, ox, oy, oz, vif, aif, pos, vel, aux, lds[w],
                                         acc);
     } else {
         const float ox = offset_f<true>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
         const float oy = offset_f<true>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
         const float oz = offset_f<true>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
-        sweep_b_pairs_f<SURF, K, true>(C, tabf, f, ti, pf, ox, oy, oz, vif, aif, pos, vel, aux, lds[w], ldsj[w],
+        sweep_b_pairs_f<SURF, K, true>(C, tabf, f, ti, pf, ox, oy, oz, vif, aif, pos, vel, aux, lds[w],
                                        acc);
     }
     acc.px = slot_sum_f<K>(acc.px); acc.py = slot_sum_f<K>(acc.py); acc.pz = slot_sum_f<K>(acc.pz);
