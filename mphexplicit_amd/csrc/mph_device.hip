@@ -1133,6 +1133,7 @@ struct mph_surrogate {
     DevBuf<bf16x8> pk;
     DevBuf<int> pcnt, pnbr;
     DevBuf<float4> apred;                 // by original id (inference) or list entry (training)
+    DevBuf<float4> msum;                  // message sums of the inference, sorted order
     // training set of the current sample
     DevBuf<int> plist, toff, towner;
     int nlist = 0, ntiles = 0;
@@ -1210,9 +1211,13 @@ static int sur_grid(const mph_ctx* c)
 static int sur_infer(mph_surrogate* s, const double4* velf)
 {
     mph_ctx* c = s->c;
+    const SurScales sc = sur_scales(s);
+    HIP_TRY(hipMemsetAsync(s->msum.p, 0, sizeof(float4) * c->n, c->stream));
     hipLaunchKernelGGL(k_sur_infer, dim3(sur_grid(c)), dim3(256), 0, c->stream, c->d_cases.p, c->cl_info.p,
-                       c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->pos.p, velf, c->vstrain.p, c->pidx.p,
-                       s->pk.p, s->w.p, sur_scales(s), s->apred.p);
+                       c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->pos.p, velf, c->vstrain.p, s->pk.p,
+                       s->w.p, sc, s->msum.p);
+    hipLaunchKernelGGL(k_sur_finish, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->d_cases.p, c->n,
+                       c->pos.p, c->pidx.p, s->msum.p, sc, s->apred.p);
     return launch_ok("surrogate forward");
 }
 
@@ -1309,6 +1314,7 @@ extern "C" mph_surrogate* mph_sur_create(mph_ctx* c, const mph_sur_params* p)
     if ((rc = s->w.alloc(kSurNP)) || (rc = s->m1.alloc(kSurNP)) || (rc = s->m2.alloc(kSurNP)) ||
         (rc = s->grad.alloc(kSurNP)) || (rc = s->wr.alloc(kSurNP)) || (rc = s->pk.alloc(kPkTiles * 64)) ||
         (rc = s->pcnt.alloc(c->n)) || (rc = s->pnbr.alloc((size_t)c->n * s->wh)) || (rc = s->apred.alloc(c->n)) ||
+        (rc = s->msum.alloc(c->n)) ||
         (rc = s->loss.alloc(1)) || (rc = s->sums.alloc(4)) || (rc = s->v_orig.alloc(c->n)) ||
         (rc = s->v_ref.alloc(c->n)) || (rc = s->n_ref.alloc(c->n)))
         return nullptr;

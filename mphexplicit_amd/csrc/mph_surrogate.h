@@ -150,12 +150,14 @@ __device__ __forceinline__ void sur_load_weights(SurW& W, const bf16x8* __restri
 // features, symmetric under i <-> j, the three layers on the matrix cores and the
 // antisymmetric message m = c0 e + c1 u + c2 (u.e) e.  Adds the tile's message sum to
 // (ax, ay, az) in every lane.  TRAIN stores X, Z1..Z3 and the basis at pair slot P.
-template <bool TRAIN>
+// SCATTER also adds m_ji = -m_ij to msum[j] (atomically) for the pairs with scat set.
+template <bool TRAIN, bool SCATTER = false>
 __device__ __forceinline__ void sur_tile(const SurW& W, const SurScales& sc, int g, bool valid, float dx, float dy,
                                          float dz, int ci, int cj, float di, float dj, float ux, float uy, float uz,
                                          size_t P, __bf16* __restrict__ X, float* __restrict__ Z1,
                                          float* __restrict__ Z2, float* __restrict__ Z3, float* __restrict__ Bs,
-                                         float& ax, float& ay, float& az)
+                                         float& ax, float& ay, float& az, float4* __restrict__ msum = nullptr,
+                                         int j = 0, bool scat = false)
 {
     float f[16];
     float ex, ey, ez, ue;
@@ -225,6 +227,11 @@ __device__ __forceinline__ void sur_tile(const SurW& W, const SurScales& sc, int
         *reinterpret_cast<float4*>(Bs + P * 8) = make_float4(ex, ey, ez, ux);
         *reinterpret_cast<float4*>(Bs + P * 8 + 4) = make_float4(uy, uz, ue, 0.f);
     }
+    if (SCATTER && g == 0 && scat) {
+        atomicAdd(&msum[j].x, -mx);
+        atomicAdd(&msum[j].y, -my);
+        atomicAdd(&msum[j].z, -mz);
+    }
     for (int off = 8; off > 0; off >>= 1) {
         mx += __shfl_xor(mx, off, 16);
         my += __shfl_xor(my, off, 16);
@@ -287,14 +294,16 @@ k_sur_forward_train(const CaseConst* __restrict__ cases, const int* __restrict__
 // Inference forward fused with the pair search: persistent wavefronts over the clusters;
 // for each fluid member the 64 lanes test 64 union-list entries at a time (exact periodic
 // difference, r <= h), ballot-compact the pairs with their FP32 offsets into this
-// wavefront's LDS queue, and every 16 queued pairs run one matrix tile.  Writes a_i by
-// original id (0 for walls).
+// wavefront's LDS queue, and every 16 queued pairs run one matrix tile.
+// Each unordered fluid pair is evaluated once, by its lower sorted index i: the features
+// are symmetric and the message antisymmetric, so m_ji = -m_ij goes to j by an atomic
+// add.  Fluid-wall pairs are evaluated by the fluid particle.  Sums go to msum (sorted
+// order, zeroed by the caller); k_sur_finish turns them into a_i.
 __global__ void __launch_bounds__(256)
 k_sur_infer(const CaseConst* __restrict__ cases, const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
             const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
             const double4* __restrict__ pos, const double4* __restrict__ velf, const double* __restrict__ strain,
-            const int* __restrict__ pidx, const bf16x8* __restrict__ pk, const float* __restrict__ w, SurScales sc,
-            float4* __restrict__ apred)
+            const bf16x8* __restrict__ pk, const float* __restrict__ w, SurScales sc, float4* __restrict__ msum)
 {
     constexpr int kQ = 80;                       // queue: 15 left over + 64 new at most
     __shared__ int qj[4][kQ];
@@ -316,10 +325,7 @@ k_sur_infer(const CaseConst* __restrict__ cases, const int4* __restrict__ cl_inf
             const int i = info.x + m;
             const double4 pi = pos[i];
             const int ti = tag_of(pi.w) & 3;
-            if (ti >= 2) {
-                if (lane == 0) apred[pidx[i]] = make_float4(0.f, 0.f, 0.f, 0.f);
-                continue;
-            }
+            if (ti >= 2) continue;
             const double4 vi = velf[i];
             const float di = (float)strain[i] * sc.inv_n0p * sc.dens_scale;
             const int ci = sur_class(ti);
@@ -332,10 +338,11 @@ k_sur_infer(const CaseConst* __restrict__ cases, const int4* __restrict__ cl_inf
                 const float4 d = qd[wv][p];
                 const double4 vj = velf[j];
                 const float dj = (float)strain[j] * sc.inv_n0p * sc.dens_scale;
-                sur_tile<false>(W, sc, g, valid, d.x, d.y, d.z, ci, __float_as_int(d.w), di, dj,
-                                (float)(vj.x - vi.x) * sc.inv_u, (float)(vj.y - vi.y) * sc.inv_u,
-                                (float)(vj.z - vi.z) * sc.inv_u, 0, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                ax, ay, az);
+                const int cj = __float_as_int(d.w);
+                sur_tile<false, true>(W, sc, g, valid, d.x, d.y, d.z, ci, cj, di, dj,
+                                      (float)(vj.x - vi.x) * sc.inv_u, (float)(vj.y - vi.y) * sc.inv_u,
+                                      (float)(vj.z - vi.z) * sc.inv_u, 0, nullptr, nullptr, nullptr, nullptr,
+                                      nullptr, ax, ay, az, msum, j, valid && cj < 2);
             };
             for (int e0 = 0; e0 < len; e0 += 64) {
                 const int e = e0 + lane;
@@ -345,9 +352,10 @@ k_sur_infer(const CaseConst* __restrict__ cases, const int4* __restrict__ cl_inf
                 int cj = 0;
                 if (j >= 0 && j != i) {
                     const double4 pj = pos[j];
-                    rel3(C, pi, pj, dxd, dyd, dzd);
-                    ok = dxd * dxd + dyd * dyd + dzd * dzd <= C.hmax2;
                     cj = sur_class(tag_of(pj.w) & 3);
+                    rel3(C, pi, pj, dxd, dyd, dzd);
+                    // fluid partners with a lower index evaluate this pair themselves
+                    ok = dxd * dxd + dyd * dyd + dzd * dzd <= C.hmax2 && (j > i || cj == 2);
                 }
                 const unsigned long long mask = __ballot(ok);
                 if (ok) {
@@ -377,12 +385,32 @@ k_sur_infer(const CaseConst* __restrict__ cases, const int4* __restrict__ cl_inf
             }
             if (count > 0) run_tile(count);
             if (lane == 0) {
-                const float rr = (float)(C.mass[0] / C.mass[ti]) * sc.a_scale;
-                apred[pidx[i]] = make_float4(ax * rr, ay * rr, az * rr, rr);
+                atomicAdd(&msum[i].x, ax);
+                atomicAdd(&msum[i].y, ay);
+                atomicAdd(&msum[i].z, az);
             }
             __builtin_amdgcn_wave_barrier();
         }
     }
+}
+
+// a_i = a_scale rho_0/rho_i sum_j m_ij by original id (0 for walls), from the sums of
+// k_sur_infer; .w = a_scale rho_0/rho_i
+__global__ void __launch_bounds__(256)
+k_sur_finish(const CaseConst* __restrict__ cases, int n, const double4* __restrict__ pos,
+             const int* __restrict__ pidx, const float4* __restrict__ msum, SurScales sc, float4* __restrict__ apred)
+{
+    const CaseConst& C = cases[0];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ti = tag_of(pos[i].w) & 3;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ti < 2) {
+        const float rr = (float)(C.mass[0] / C.mass[ti]) * sc.a_scale;
+        const float4 m = msum[i];
+        a = make_float4(m.x * rr, m.y * rr, m.z * rr, rr);
+    }
+    apred[pidx[i]] = a;
 }
 
 // loss over the training list and dL/dm per particle (FP32), with the loss sum in FP64
