@@ -1203,7 +1203,7 @@ static int sur_grid(const mph_ctx* c)
     hipDeviceProp_t prop;
     int cus = 256;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) cus = prop.multiProcessorCount;
-    return cus * 4;     // persistent: four 256-thread blocks per CU
+    return cus * 5;     // persistent: five 256-thread blocks per CU (the LDS and register limit)
 }
 
 // inference forward at (pos, velf, vstrain) over all fluid particles, pair search fused:
