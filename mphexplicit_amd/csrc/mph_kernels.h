@@ -708,6 +708,9 @@ __device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restric
         }
         const int rest = len - (blk << 6);
         const int nsteps = rest >= 64 ? K : (rest + S - 1) / S;
+        // not unrolled: one pair per iteration keeps the sweeps within 80 VGPRs (six
+        // wavefronts per SIMD), which measures faster than the unrolled forms
+#pragma unroll 1
         for (int t = 0; t < nsteps; ++t) {
             const int e = t * S + f.slot;
             Rec q;
@@ -799,15 +802,14 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
     walk_union_lds<K, 2>(f, lds, fetch, eval);
 }
 
-// Without surface tension both sweeps are held to five wavefronts per SIMD (at most 96
-// VGPRs; a few bytes are spilled outside the pair loop).  Left alone the compiler takes 100
-// for the straight-line pair loop, which leaves four wavefronts and measures slower.  Six
-// (80 VGPRs) spills inside the block loop.
+// Without surface tension both sweeps run at six wavefronts per SIMD (at most 80 VGPRs;
+// sweep A needs 78, sweep B spills a few bytes outside the pair loop), measured faster than
+// five.
 // COUNT: also count the interacting pairs (r <= largest radius) into pair_count; only the
 // timed steps that report pair rates use it.  SAMEAP: the attractive and pressure kernels
 // have the same radius (one weight per pair)
 template <int SURF, int K, bool COUNT, bool SAMEAP>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 5, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 6, 8)))
 k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
           const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
@@ -962,9 +964,9 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
 }
 
 // SAMEPV: the pressure and viscosity kernels have the same radius (one weight per pair).
-// Five wavefronts per SIMD without surface tension, as sweep A.
+// Six wavefronts per SIMD without surface tension, as sweep A.
 template <int SURF, int K, bool SAMEPV>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 5, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 6, 8)))
 k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
           const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,

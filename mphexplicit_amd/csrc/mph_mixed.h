@@ -74,6 +74,7 @@ __device__ __forceinline__ void walk_union_lds_f(const Frame<K>& f, v4f* __restr
         }
         const int rest = len - (blk << 6);
         const int nsteps = rest >= 64 ? K : (rest + S - 1) / S;
+#pragma unroll 1
         for (int t = 0; t < nsteps; ++t) {
             const int e = t * S + f.slot;
             RecF q;
@@ -136,11 +137,11 @@ __device__ __forceinline__ void sweep_a_pairs_f(const CaseConst& C, const float*
     walk_union_lds_f<K, 2>(f, lds, fetch, eval);
 }
 
-// Without surface tension the mixed sweeps are held to six wavefronts per SIMD (80 VGPRs,
-// some spilled outside the pair loop): measured faster than five (96 VGPRs) and than the
-// compiler's choice (99 for sweep B, four wavefronts).
+// Without surface tension the mixed sweeps are held to seven wavefronts per SIMD (72
+// VGPRs, a few bytes spilled outside the pair loop): measured faster than six and five and
+// than the compiler's choice (four).
 template <int SURF, int K>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 6, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 7, 8)))
 k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
                 const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
                 const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
@@ -307,7 +308,7 @@ __device__ __forceinline__ float slot_sum_f(float v)
 }
 
 template <int SURF, int K>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 6, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 7, 8)))
 k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
                 const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
                 const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
