@@ -186,11 +186,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-case batch leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-step leg")
     ap.add_argument("--sur-warm", type=int, default=100)
-    ap.add_argument("--sur-samples", type=int, default=60)
+    ap.add_argument("--sur-samples", type=int, default=120)
     ap.add_argument("--sur-iters", type=int, default=8)
     ap.add_argument("--sur-stride", type=int, default=32)
     ap.add_argument("--sur-steps", type=int, default=10)
-    ap.add_argument("--sur-blocks", type=int, default=4)
+    ap.add_argument("--sur-blocks", type=int, default=8)
     ap.add_argument("--sur-k", type=int, default=10)
     ap.add_argument("--sur-tol", type=float, default=0.01)
     ap.add_argument("--error-steps", type=int, default=500,
