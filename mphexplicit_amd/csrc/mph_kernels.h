@@ -527,66 +527,139 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
     if (r == C.tcc - 1) cl_count[c] = base + nq;
 }
 
-// union lists: thread (cluster slot, member); the K member lanes of a cluster walk the
-// same candidates in lockstep and vote with a ballot
+// Union lists, one wavefront per cluster (main.cpp:1329-1596 per member, united in ascending
+// index).  The candidates are the particles of the cluster's 3^d stencil cells in ascending
+// rank and index; the wavefront stages 64 of them at a time in LDS (one load per lane, the next
+// block's loads in flight) and tests them as K rows (members) x 64/K slots with the
+// reference's cut-off test.  A candidate joins the union if some member has it within the
+// cut-off and is not it; the union keeps candidate order.  Per member: NeighborCount as the
+// reference counts it (main.cpp:1408-1420).  A lane walking its candidates with one
+// dependent load each made the list build latency-bound when only a few clusters rebuild.
+template <int K>
 __global__ void __launch_bounds__(256)
 k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slot_case, int nslots,
                 const int* __restrict__ flags, const int* __restrict__ any_flag,
                 const double4* __restrict__ pos, const int* __restrict__ start,
                 const int* __restrict__ rank, const int4* __restrict__ cl_info,
-                const int* __restrict__ cl_count, int K, int width, int* __restrict__ cl_nbr,
+                const int* __restrict__ cl_count, int width, int* __restrict__ cl_nbr,
                 int* __restrict__ cl_len, int* __restrict__ count_out, int* __restrict__ max_len)
 {
     if (*any_flag == 0) return;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int g = (int)(t / K), m = (int)(t % K);
-    int ulen = 0;
-    bool run = g < nslots;
-    int c = 0;
-    if (run) {
-        c = slot_case[g];
-        run = flags[c] != 0 && g - cases[c].cl_base < cl_count[c];
+    constexpr int S = 64 / K;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ double4 cand[4][64];
+    __shared__ int candj[4][64];         // particle index, -1 past the end
+    __shared__ int candf[4][64];         // 1: in the fluid block of its cell
+    __shared__ int srank[4][32], clo[4][32], cmid[4][32], cpre[4][33];
+    const int g = blockIdx.x * 4 + wv;
+    if (g >= nslots) return;                                 // wave-uniform from here on
+    const int c = slot_case[g];
+    if (flags[c] == 0 || g - cases[c].cl_base >= cl_count[c]) return;
+    const CaseConst& C = cases[c];
+    const int4 info = cl_info[g];
+    const int row = lane / S, slot = lane % S;
+    const bool mine = row < info.y;
+    const int i = info.x + (mine ? row : 0);
+    const double4 pi = pos[i];
+    const bool wall_i = (tag_of(pi.w) & 3) >= 2;
+    // stencil cells: lane q < nc takes the q-th cell of the window, then its place in
+    // ascending rank
+    const int cid = info.z;
+    const int iz = cid % C.cc[2];
+    const int iy = (cid / C.cc[2]) % C.cc[1];
+    const int ix = cid / (C.cc[1] * C.cc[2]);
+    int wx[3], wy[3], wz[3];
+    const int nx = axis_window(ix, C.cc[0], 1, wx);
+    const int ny = axis_window(iy, C.cc[1], 1, wy);
+    const int nz = axis_window(iz, C.cc[2], C.dim == 2 ? 0 : 1, wz);
+    const int nc = nx * ny * nz;
+    int v = 0;
+    if (lane < nc) {
+        const int a = lane / (ny * nz), b = (lane / nz) % ny, d = lane % nz;
+        v = rank[C.rank_base + (wx[a] * C.cc[1] + wy[b]) * C.cc[2] + wz[d]];
+        srank[wv][lane] = v;
     }
-    if (run) {
-        const CaseConst& C = cases[c];
-        const int4 info = cl_info[g];
-        const bool mine = m < info.y;
-        const int i = info.x + (mine ? m : 0);
-        const double4 pi = pos[i];
-        const bool wall_i = (tag_of(pi.w) & 3) >= 2;
-        int rk[27];
-        const int nc = stencil_ranks(C, rank, info.z, rk);
-        const int lane = threadIdx.x & 63;
-        const int seg = lane / K * K;
-        const unsigned long long segmask = (K == 64 ? ~0ull : ((1ull << K) - 1ull)) << seg;
-        int count = 0, nfluid = 0, self = 0;
-        int* out = cl_nbr + (size_t)g * width;
-        for (int q = 0; q < nc; ++q) {
-            const int k0 = C.key_base + 2 * rk[q];
-            const int lo = start[k0], mid = start[k0 + 1], hi = start[k0 + 2];
-            for (int j = lo; j < hi; ++j) {
-                const double4 pj = pos[j];
-                double dx, dy, dz;
-                rel3(C, pi, pj, dx, dy, dz);
-                const double q2 = dx * dx + dy * dy + dz * dz;
-                const bool in = mine && q2 <= C.cut2;
-                const bool other = in && j != i;
-                self |= in && j == i;
-                count += other;
-                nfluid += other && j < mid;
-                const unsigned long long vote = __ballot(other) & segmask;
-                if (vote) {
-                    if (m == 0 && ulen < width) out[ulen] = j;
-                    ++ulen;
-                }
-            }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < nc) {
+        int at = 0;
+        for (int q = 0; q < nc; ++q) at += srank[wv][q] < v;
+        const int k0 = C.key_base + 2 * v;
+        const int lo = start[k0];
+        clo[wv][at] = lo;
+        cmid[wv][at] = start[k0 + 1];
+        cpre[wv][at + 1] = start[k0 + 2] - lo;
+    }
+    if (lane == 0) cpre[wv][0] = 0;
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0)
+        for (int q = 0; q < nc; ++q) cpre[wv][q + 1] += cpre[wv][q];
+    __builtin_amdgcn_wave_barrier();
+    const int total = cpre[wv][nc];
+    // candidate e -> particle index and fluid flag; the cells of one block are consecutive
+    // from cell q0
+    int q0 = 0;
+    auto locate = [&](int e, int& fl) {
+        int q = q0;
+        while (q + 1 < nc && cpre[wv][q + 1] <= e) ++q;
+        const int j = clo[wv][q] + (e - cpre[wv][q]);
+        fl = j < cmid[wv][q];
+        return j;
+    };
+    auto advance = [&](int e) {
+        while (q0 + 1 < nc && cpre[wv][q0 + 1] <= e) ++q0;
+    };
+    int count = 0, nfluid = 0, self = 0, ulen = 0;
+    int* out = cl_nbr + (size_t)g * width;
+    int fn = 0;
+    int jn = lane < total ? locate(lane, fn) : -1;
+    double4 pn = jn >= 0 ? pos[jn] : make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int base = 0; base < total; base += 64) {
+        __builtin_amdgcn_wave_barrier();
+        cand[wv][lane] = pn;
+        candj[wv][lane] = jn;
+        candf[wv][lane] = fn;
+        __builtin_amdgcn_wave_barrier();
+        const int nb = min(64, total - base);
+        if (base + 64 < total) {
+            advance(base + 64);
+            const int e = base + 64 + lane;
+            jn = e < total ? locate(e, fn) : -1;
+            if (jn >= 0) pn = pos[jn];
         }
-        if (mine) count_out[i] = (wall_i && nfluid == 0) ? 0 : count + self;
-        if (m == 0) cl_len[g] = ulen < width ? ulen : width;
+        const int nsteps = (nb + S - 1) / S;
+        for (int t = 0; t < nsteps; ++t) {
+            const int e = t * S + slot;
+            const int j = candj[wv][e];
+            const double4 pj = cand[wv][e];
+            double dx, dy, dz;
+            rel3(C, pi, pj, dx, dy, dz);
+            const double q2 = dx * dx + dy * dy + dz * dz;
+            const bool in = mine && e < nb && q2 <= C.cut2;
+            const bool other = in && j != i;
+            self |= in && j == i;
+            count += other;
+            nfluid += other && candf[wv][e];
+            const unsigned long long b = __ballot(other);
+            unsigned long long u = 0;
+            for (int r = 0; r < K; ++r) u |= b >> (r * S);
+            u &= (S == 64 ? ~0ull : ((1ull << S) - 1ull));
+            if (row == 0 && ((u >> slot) & 1ull)) {
+                const int at = ulen + (int)__popcll(u & ((1ull << slot) - 1ull));
+                if (at < width) out[at] = j;
+            }
+            ulen += (int)__popcll(u);
+        }
     }
-    int mx = ulen;
-    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(max_len, mx);
+    for (int off = S / 2; off > 0; off >>= 1) {
+        count += __shfl_xor(count, off, S);
+        nfluid += __shfl_xor(nfluid, off, S);
+        self |= __shfl_xor(self, off, S);
+    }
+    if (mine && slot == 0) count_out[i] = (wall_i && nfluid == 0) ? 0 : count + self;
+    if (lane == 0) {
+        cl_len[g] = ulen < width ? ulen : width;
+        atomicMax(max_len, ulen);
+    }
 }
 
 // -------------------------------------------------------------------------------------
