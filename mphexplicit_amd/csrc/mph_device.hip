@@ -458,7 +458,10 @@ static int launch_rebuild(mph_ctx* c, int count_only)
     hipLaunchKernelGGL(k_cluster_fill, dim3(gcells), dim3(256), 0, s, c->d_cases.p, c->cell_case.p,
                        c->ncells, c->start.p, c->cell_of_rank.p, c->K, c->ccount.p, c->cl_info.p,
                        c->cl_count.p, c->any_flag.p);
-    LAUNCH_K(k_cluster_lists, c->K, dim3((unsigned)((c->nslots + 3) / 4)), dim3(256), 0, s, c->d_cases.p,
+    // persistent 4-wavefront blocks, at most 4096 (four resident rounds at 120 VGPRs): a step
+    // without a rebuild exits in a few microseconds; fewer blocks measured slower
+    const int lgrid = std::max(1, std::min((c->nslots + 3) / 4, 4096));
+    LAUNCH_K(k_cluster_lists, c->K, dim3((unsigned)lgrid), dim3(256), 0, s, c->d_cases.p,
              c->slot_case.p, c->nslots, c->flags.p, c->any_flag.p, c->pos.p, c->start.p, c->rank.p,
              c->cl_info.p, c->cl_count.p, count_only ? 0 : c->width, c->cl_nbr.p, c->cl_len.p, c->nout.p,
              c->max_len.p);

@@ -64,6 +64,19 @@ __device__ __forceinline__ void rel3(const CaseConst& C, const double4& pi, cons
     dz = mod_w(pj.z - pi.z + C.hwid[2], C.dwid[2]) - C.hwid[2];
 }
 
+// rel3 on one axis with the common case branch-free: for a = x_j - x_i + W/2 in [-W, 2W)
+// (positions inside the box, which the move keeps) the shift is -W, 0 or W and one
+// subtraction rounds like the reference; a - 0.0 differs from mod_w's a + 0.0 only in the
+// sign of a zero, which the following - W/2 removes.  Other a take the division, in lanes
+// that need it only.
+__device__ __forceinline__ double rel_exact(double xj, double xi, double hw, double w)
+{
+    const double a = xj - xi + hw;
+    double m = a - (a < 0.0 ? -w : (a >= w ? w : 0.0));
+    if (!(a >= -w && a < w + w)) m = a - w * floor(a / w);
+    return m - hw;
+}
+
 // The same difference for a list partner, without the division: with positions inside
 // the periodic box, a = x_j - x_i + W/2 lies in [-W, 2W), where floor(a/W) is -1, 0 or 1.
 __device__ __forceinline__ double rel_wrap(double xj, double xi, double hw, double w)
@@ -529,39 +542,60 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
 
 // Union lists, one wavefront per cluster (main.cpp:1329-1596 per member, united in ascending
 // index).  The candidates are the particles of the cluster's 3^d stencil cells in ascending
-// rank and index; the wavefront stages 64 of them at a time in LDS (one load per lane, the next
-// block's loads in flight) and tests them as K rows (members) x 64/K slots with the
-// reference's cut-off test.  A candidate joins the union if some member has it within the
-// cut-off and is not it; the union keeps candidate order.  Per member: NeighborCount as the
-// reference counts it (main.cpp:1408-1420).  A lane walking its candidates with one
-// dependent load each made the list build latency-bound when only a few clusters rebuild.
+// rank and index.  The wavefront takes 64 at a time (one load per lane, the next block's
+// loads in flight), drops those farther than the cut-off from the members' bounding box
+// (conservatively: cut-off^2 (1 + 1e-9) against a box built from the same exact
+// differences), compacts the rest into LDS and tests them as K rows (members) x 64/K slots
+// with the reference's cut-off test.  A candidate joins the union if some member has it within
+// the cut-off and is not it; the union keeps candidate order.  Per member: NeighborCount as
+// the reference counts it (main.cpp:1408-1420).  Persistent wavefronts: a step without a
+// rebuild costs a small grid, and a rebuild of a few clusters runs them in parallel.  One
+// thread per member walking the candidates with a dependent load each measured 5.5-6 ms per
+// full rebuild of the 1.27M-particle deck (this kernel: 4-4.7 ms) and was latency-bound when
+// few clusters rebuild.
+
 template <int K>
-__global__ void __launch_bounds__(256)
-k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slot_case, int nslots,
-                const int* __restrict__ flags, const int* __restrict__ any_flag,
-                const double4* __restrict__ pos, const int* __restrict__ start,
-                const int* __restrict__ rank, const int4* __restrict__ cl_info,
-                const int* __restrict__ cl_count, int width, int* __restrict__ cl_nbr,
-                int* __restrict__ cl_len, int* __restrict__ count_out, int* __restrict__ max_len)
+__device__ __forceinline__ void cluster_list_one(const CaseConst& C, int g, const int4 info,
+                                                 const double4* __restrict__ pos, const int* __restrict__ start,
+                                                 const int* __restrict__ rank, int width, int* __restrict__ cl_nbr,
+                                                 int* __restrict__ cl_len, int* __restrict__ count_out,
+                                                 int* __restrict__ max_len, double4* cand, int* candj, int* candf,
+                                                 int* srank, int* clo, int* cmid, int* cpre)
 {
-    if (*any_flag == 0) return;
     constexpr int S = 64 / K;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    __shared__ double4 cand[4][64];
-    __shared__ int candj[4][64];         // particle index, -1 past the end
-    __shared__ int candf[4][64];         // 1: in the fluid block of its cell
-    __shared__ int srank[4][32], clo[4][32], cmid[4][32], cpre[4][33];
-    const int g = blockIdx.x * 4 + wv;
-    if (g >= nslots) return;                                 // wave-uniform from here on
-    const int c = slot_case[g];
-    if (flags[c] == 0 || g - cases[c].cl_base >= cl_count[c]) return;
-    const CaseConst& C = cases[c];
-    const int4 info = cl_info[g];
+    const int lane = threadIdx.x & 63;
     const int row = lane / S, slot = lane % S;
     const bool mine = row < info.y;
     const int i = info.x + (mine ? row : 0);
     const double4 pi = pos[i];
     const bool wall_i = (tag_of(pi.w) & 3) >= 2;
+    // members' bounding box, relative to the first member
+    const double4 pf = pos[info.x];
+    double bx0 = rel_exact(pi.x, pf.x, C.hwid[0], C.dwid[0]), bx1 = bx0;
+    double by0 = rel_exact(pi.y, pf.y, C.hwid[1], C.dwid[1]), by1 = by0;
+    double bz0 = rel_exact(pi.z, pf.z, C.hwid[2], C.dwid[2]), bz1 = bz0;
+    for (int off = 32; off > 0; off >>= 1) {
+        bx0 = fmin(bx0, __shfl_xor(bx0, off)); bx1 = fmax(bx1, __shfl_xor(bx1, off));
+        by0 = fmin(by0, __shfl_xor(by0, off)); by1 = fmax(by1, __shfl_xor(by1, off));
+        bz0 = fmin(bz0, __shfl_xor(bz0, off)); bz1 = fmax(bz1, __shfl_xor(bz1, off));
+    }
+    const double bound = C.cut2 * (1.0 + 1e-9);
+    // the box test needs one minimum image for all members: |x_j - x_first| < W/2 on every
+    // axis for candidates within the cut-off of a member (members share a cell narrower than
+    // two cut-offs), which eight cut-offs per width guarantee; narrower boxes test everything
+    const double w8 = 64.0 * C.cut2;
+    const bool use_box = C.dwid[0] * C.dwid[0] >= w8 && C.dwid[1] * C.dwid[1] >= w8 &&
+                         (C.dim == 2 || C.dwid[2] * C.dwid[2] >= w8);
+    auto near_box = [&](const double4& p) {
+        if (!use_box) return true;
+        const double dx = rel_exact(p.x, pf.x, C.hwid[0], C.dwid[0]);
+        const double dy = rel_exact(p.y, pf.y, C.hwid[1], C.dwid[1]);
+        const double dz = rel_exact(p.z, pf.z, C.hwid[2], C.dwid[2]);
+        const double ex = fmax(fmax(dx - bx1, bx0 - dx), 0.0);
+        const double ey = fmax(fmax(dy - by1, by0 - dy), 0.0);
+        const double ez = fmax(fmax(dz - bz1, bz0 - dz), 0.0);
+        return ex * ex + ey * ey + ez * ez <= bound;
+    };
     // stencil cells: lane q < nc takes the q-th cell of the window, then its place in
     // ascending rank
     const int cid = info.z;
@@ -577,71 +611,76 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
     if (lane < nc) {
         const int a = lane / (ny * nz), b = (lane / nz) % ny, d = lane % nz;
         v = rank[C.rank_base + (wx[a] * C.cc[1] + wy[b]) * C.cc[2] + wz[d]];
-        srank[wv][lane] = v;
+        srank[lane] = v;
     }
     __builtin_amdgcn_wave_barrier();
     if (lane < nc) {
         int at = 0;
-        for (int q = 0; q < nc; ++q) at += srank[wv][q] < v;
+        for (int q = 0; q < nc; ++q) at += srank[q] < v;
         const int k0 = C.key_base + 2 * v;
         const int lo = start[k0];
-        clo[wv][at] = lo;
-        cmid[wv][at] = start[k0 + 1];
-        cpre[wv][at + 1] = start[k0 + 2] - lo;
+        clo[at] = lo;
+        cmid[at] = start[k0 + 1];
+        cpre[at + 1] = start[k0 + 2] - lo;
     }
-    if (lane == 0) cpre[wv][0] = 0;
+    if (lane == 0) cpre[0] = 0;
     __builtin_amdgcn_wave_barrier();
     if (lane == 0)
-        for (int q = 0; q < nc; ++q) cpre[wv][q + 1] += cpre[wv][q];
+        for (int q = 0; q < nc; ++q) cpre[q + 1] += cpre[q];
     __builtin_amdgcn_wave_barrier();
-    const int total = cpre[wv][nc];
+    const int total = cpre[nc];
     // candidate e -> particle index and fluid flag; the cells of one block are consecutive
     // from cell q0
     int q0 = 0;
     auto locate = [&](int e, int& fl) {
         int q = q0;
-        while (q + 1 < nc && cpre[wv][q + 1] <= e) ++q;
-        const int j = clo[wv][q] + (e - cpre[wv][q]);
-        fl = j < cmid[wv][q];
+        while (q + 1 < nc && cpre[q + 1] <= e) ++q;
+        const int j = clo[q] + (e - cpre[q]);
+        fl = j < cmid[q];
         return j;
     };
-    auto advance = [&](int e) {
-        while (q0 + 1 < nc && cpre[wv][q0 + 1] <= e) ++q0;
-    };
+    const unsigned long long below = (1ull << lane) - 1ull;
     int count = 0, nfluid = 0, self = 0, ulen = 0;
     int* out = cl_nbr + (size_t)g * width;
     int fn = 0;
     int jn = lane < total ? locate(lane, fn) : -1;
     double4 pn = jn >= 0 ? pos[jn] : make_double4(0.0, 0.0, 0.0, 0.0);
     for (int base = 0; base < total; base += 64) {
+        // compact this block's candidates near the box into LDS, in candidate order
+        const bool kp = jn >= 0 && near_box(pn);
+        const unsigned long long kb = __ballot(kp);
+        const int ns = (int)__popcll(kb);
         __builtin_amdgcn_wave_barrier();
-        cand[wv][lane] = pn;
-        candj[wv][lane] = jn;
-        candf[wv][lane] = fn;
+        if (kp) {
+            const int at = (int)__popcll(kb & below);
+            cand[at] = pn;
+            candj[at] = jn;
+            candf[at] = fn;
+        }
         __builtin_amdgcn_wave_barrier();
-        const int nb = min(64, total - base);
         if (base + 64 < total) {
-            advance(base + 64);
+            while (q0 + 1 < nc && cpre[q0 + 1] <= base + 64) ++q0;
             const int e = base + 64 + lane;
             jn = e < total ? locate(e, fn) : -1;
             if (jn >= 0) pn = pos[jn];
         }
-        const int nsteps = (nb + S - 1) / S;
+        const int nsteps = (ns + S - 1) / S;
         for (int t = 0; t < nsteps; ++t) {
             const int e = t * S + slot;
-            const int j = candj[wv][e];
-            const double4 pj = cand[wv][e];
-            double dx, dy, dz;
-            rel3(C, pi, pj, dx, dy, dz);
+            const int j = candj[e];
+            const double4 pj = cand[e];
+            const double dx = rel_exact(pj.x, pi.x, C.hwid[0], C.dwid[0]);
+            const double dy = rel_exact(pj.y, pi.y, C.hwid[1], C.dwid[1]);
+            const double dz = rel_exact(pj.z, pi.z, C.hwid[2], C.dwid[2]);
             const double q2 = dx * dx + dy * dy + dz * dz;
-            const bool in = mine && e < nb && q2 <= C.cut2;
+            const bool in = mine && e < ns && q2 <= C.cut2;
             const bool other = in && j != i;
             self |= in && j == i;
             count += other;
-            nfluid += other && candf[wv][e];
-            const unsigned long long b = __ballot(other);
+            nfluid += other && candf[e];
+            const unsigned long long bv = __ballot(other);
             unsigned long long u = 0;
-            for (int r = 0; r < K; ++r) u |= b >> (r * S);
+            for (int r = 0; r < K; ++r) u |= bv >> (r * S);
             u &= (S == 64 ? ~0ull : ((1ull << S) - 1ull));
             if (row == 0 && ((u >> slot) & 1ull)) {
                 const int at = ulen + (int)__popcll(u & ((1ull << slot) - 1ull));
@@ -659,6 +698,30 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
     if (lane == 0) {
         cl_len[g] = ulen < width ? ulen : width;
         atomicMax(max_len, ulen);
+    }
+    __builtin_amdgcn_wave_barrier();       // this cluster's LDS reads precede the next one's writes
+}
+
+template <int K>
+__global__ void __launch_bounds__(256)
+k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slot_case, int nslots,
+                const int* __restrict__ flags, const int* __restrict__ any_flag,
+                const double4* __restrict__ pos, const int* __restrict__ start,
+                const int* __restrict__ rank, const int4* __restrict__ cl_info,
+                const int* __restrict__ cl_count, int width, int* __restrict__ cl_nbr,
+                int* __restrict__ cl_len, int* __restrict__ count_out, int* __restrict__ max_len)
+{
+    if (*any_flag == 0) return;
+    __shared__ double4 cand[4][64];
+    __shared__ int candj[4][64];         // particle index
+    __shared__ int candf[4][64];         // 1: in the fluid block of its cell
+    __shared__ int srank[4][32], clo[4][32], cmid[4][32], cpre[4][33];
+    const int wv = threadIdx.x >> 6;
+    for (int g = blockIdx.x * 4 + wv; g < nslots; g += gridDim.x * 4) {
+        const int c = slot_case[g];
+        if (flags[c] == 0 || g - cases[c].cl_base >= cl_count[c]) continue;     // wave-uniform
+        cluster_list_one<K>(cases[c], g, cl_info[g], pos, start, rank, width, cl_nbr, cl_len, count_out, max_len,
+                            cand[wv], candj[wv], candf[wv], srank[wv], clo[wv], cmid[wv], cpre[wv]);
     }
 }
 
