@@ -83,6 +83,7 @@ struct mph_ctx {
     int mixed = 0;                 // MPH_MIXED: FP32 pair sweeps (all cases of a context)
     int same_pv = 0;               // every case has equal pressure and viscosity radii
     int same_ap = 0;               // every case has equal attractive and pressure radii
+    int ratio_one = 0;             // every case has all InteractionRatio == 1
     int K = 4;                 // cluster size (rows per wavefront)
     int width = 0;             // reserved entries per cluster list
     int nslots = 0;            // cluster slots over all cases (capacity)
@@ -479,11 +480,25 @@ static int launch_sweep_a(mph_ctx* c, int real, bool count = false)
                   real ? c->ncases : 0, real ? c->pairs.p : c->pairs_scratch.p);
         return launch_ok("sweep A");
     }
-    LAUNCH_SKCC(k_sweep_a, c->any_surface, c->K, count && real, c->same_ap, dim3(c->ncblocks()), dim3(256), 0,
-                c->stream, c->d_cases.p,
-              c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p,
-              c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p, c->wallc.p,
-              real ? c->ncases : 0, real ? c->pairs.p : c->pairs_scratch.p);
+    const int mode = (c->same_ap ? 1 : 0) | (c->ratio_one ? 2 : 0);
+#define SWEEP_A_ARGS                                                                                        \
+    dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p,  \
+        c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p,          \
+        c->vstrain.p, c->divp.p, c->wallc.p, real ? c->ncases : 0, real ? c->pairs.p : c->pairs_scratch.p
+    const bool cnt = count && real;
+#define SWEEP_A_MODE(M)                                                                            \
+    do {                                                                                           \
+        if (cnt) LAUNCH_SK__(k_sweep_a, c->any_surface, c->K, true, M, SWEEP_A_ARGS);              \
+        else LAUNCH_SK__(k_sweep_a, c->any_surface, c->K, false, M, SWEEP_A_ARGS);                 \
+    } while (0)
+    switch (mode) {
+    case 0: SWEEP_A_MODE(0); break;
+    case 1: SWEEP_A_MODE(1); break;
+    case 2: SWEEP_A_MODE(2); break;
+    default: SWEEP_A_MODE(3); break;
+    }
+#undef SWEEP_A_MODE
+#undef SWEEP_A_ARGS
     return launch_ok("sweep A");
 }
 
@@ -714,6 +729,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     c->any_surface = 0;
     c->same_pv = 1;
     c->same_ap = 1;
+    c->ratio_one = 1;
     c->mixed = prm[0].precision == MPH_MIXED;
     for (int k = 0; k < ncases; ++k) {
         const int nk = counts[k];
@@ -738,6 +754,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         c->any_surface |= ch.k.surface;
         if (ch.k.inv_hp != ch.k.inv_hv) c->same_pv = 0;
         if (ch.k.inv_ha != ch.k.inv_hp) c->same_ap = 0;
+        if (!ch.k.ratio_one) c->ratio_one = 0;
         for (int t = 0; t < kTypes; ++t)
             for (int d = 0; d < 3; ++d) hwall[((size_t)k * kTypes + t) * 3 + d] = prm[k].wall_center[t][d];
         for (int i = 0; i < nk; ++i) {

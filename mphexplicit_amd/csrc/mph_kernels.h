@@ -803,7 +803,9 @@ struct Rec {
 // block's loads are in flight while the current block is evaluated.  Entries are evaluated
 // without a branch: padding slots hold the far record and a row's own entry is caught by
 // far_if_self in the evaluators, so the loop over a block is straight-line code.
-template <int K, int NREC, class Fetch, class Eval>
+// NREC = 3 adds the gravity-centre record (surface tension).  PL3: records without the tag
+// and P, in three planes (x, y | vx, vy | z, vz): 48 bytes of LDS per entry instead of 64.
+template <int K, int NREC, class Fetch, class Eval, bool PL3 = false>
 __device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restrict__ lds, Fetch fetch, Eval eval)
 {
     // LDS slice: 2*NREC planes of 64 16-byte halves (x,y | z,w of each record), so the
@@ -825,10 +827,16 @@ __device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restric
         // before the reads below; the barriers keep the compiler from moving LDS
         // accesses across the block boundary
         __builtin_amdgcn_wave_barrier();
-        lds[0 * 64 + lane] = r.p.xy;
-        lds[1 * 64 + lane] = r.p.zw;
-        lds[2 * 64 + lane] = r.v.xy;
-        lds[3 * 64 + lane] = r.v.zw;
+        if (PL3) {
+            lds[0 * 64 + lane] = r.p.xy;
+            lds[1 * 64 + lane] = r.v.xy;
+            lds[2 * 64 + lane] = v2d{r.p.z, r.v.z};
+        } else {
+            lds[0 * 64 + lane] = r.p.xy;
+            lds[1 * 64 + lane] = r.p.zw;
+            lds[2 * 64 + lane] = r.v.xy;
+            lds[3 * 64 + lane] = r.v.zw;
+        }
         if (NREC > 2) {
             lds[4 * 64 + lane] = r.a.xy;
             lds[5 * 64 + lane] = r.a.zw;
@@ -850,10 +858,20 @@ __device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restric
         for (int t = 0; t < nsteps; ++t) {
             const int e = t * S + f.slot;
             Rec q;
-            q.p.xy = lds[0 * 64 + e];
-            q.p.zw = lds[1 * 64 + e];
-            q.v.xy = lds[2 * 64 + e];
-            q.v.zw = lds[3 * 64 + e];
+            if (PL3) {
+                q.p.xy = lds[0 * 64 + e];
+                q.v.xy = lds[1 * 64 + e];
+                const v2d zz = lds[2 * 64 + e];
+                q.p.z = zz.x;
+                q.v.z = zz.y;
+                q.p.w = 0.0;
+                q.v.w = 0.0;
+            } else {
+                q.p.xy = lds[0 * 64 + e];
+                q.p.zw = lds[1 * 64 + e];
+                q.v.xy = lds[2 * 64 + e];
+                q.v.zw = lds[3 * 64 + e];
+            }
             if (NREC > 2) {
                 q.a.xy = lds[4 * 64 + e];
                 q.a.zw = lds[5 * 64 + e];
@@ -900,12 +918,14 @@ struct AccA {
     int np;
 };
 
-template <int SURF, int K, bool WRAP, bool COUNT, bool SAMEAP>
+template <int SURF, int K, bool WRAP, bool COUNT, int MODE>
 __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, double vix, double viy, double viz,
                                               const double4* __restrict__ pos, const double4* vel,
                                               v2d* lds, AccA& acc)
 {
+    constexpr bool SAMEAP = (MODE & 1) != 0;
+    constexpr bool R1 = (MODE & 2) != 0 && !SURF;     // every InteractionRatio is 1: no tag needed
     auto fetch = [&](int j, Rec& r) {
         r.p = ld4(pos + j);
         // P (vel.w) of partners is being written by this kernel: read x, y, z only
@@ -922,7 +942,7 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         const double r = q2 * ir;
         const double sa = r * C.inv_ha;
         const double oma = fmax(1.0 - sa, 0.0);
-        const double ratio = C.ratio_one ? 1.0 : tab[16 + ti * 4 + (tag_of(q.p.w) & 3)];
+        const double ratio = (R1 || C.ratio_one) ? 1.0 : tab[16 + ti * 4 + (tag_of(q.p.w) & 3)];
         const double wa = ratio * (oma * oma);
         acc.na = fma(wa, sa, acc.na);
         if (SURF) {
@@ -935,17 +955,18 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         const double ud = fma(q.v.z - viz, dz, fma(q.v.y - viy, dy, (q.v.x - vix) * dx));
         acc.ds = fma(ud, omp * ir, acc.ds);
     };
-    walk_union_lds<K, 2>(f, lds, fetch, eval);
+    walk_union_lds<K, 2, decltype(fetch), decltype(eval), R1>(f, lds, fetch, eval);
 }
 
 // Without surface tension both sweeps run at six wavefronts per SIMD (at most 80 VGPRs;
 // sweep A needs 78, sweep B spills a few bytes outside the pair loop), measured faster than
-// five.
+// five; sweep A with three-plane records (MODE bit 1) runs at seven (72 VGPRs).
 // COUNT: also count the interacting pairs (r <= largest radius) into pair_count; only the
-// timed steps that report pair rates use it.  SAMEAP: the attractive and pressure kernels
-// have the same radius (one weight per pair)
-template <int SURF, int K, bool COUNT, bool SAMEAP>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 6, 8)))
+// timed steps that report pair rates use it.  MODE bit 0: the attractive and pressure
+// kernels have the same radius (one weight per pair); bit 1: every InteractionRatio is 1, so
+// without surface tension the partner records need no tag (three LDS planes)
+template <int SURF, int K, bool COUNT, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : ((MODE & 2) ? 7 : 6), 8)))
 k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
           const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
@@ -982,10 +1003,10 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     AccA acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
     // a wavefront takes the periodic path if any of its rows needs it (uniform branch)
     if (__ballot(!away(C, pi)) == 0ull)
-        sweep_a_pairs<SURF, K, false, COUNT, SAMEAP>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w],
+        sweep_a_pairs<SURF, K, false, COUNT, MODE>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w],
                                                      acc);
     else
-        sweep_a_pairs<SURF, K, true, COUNT, SAMEAP>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w],
+        sweep_a_pairs<SURF, K, true, COUNT, MODE>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w],
                                                     acc);
     acc.na = slot_sum<K>(acc.na);
     acc.ws = slot_sum<K>(acc.ws);
