@@ -1071,6 +1071,7 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
 {
     const double mu_i = tab[ti * 4 + ti];
     const double kp = C.dcp * C.vol, kv = C.visc_c * (-C.dcv) * C.vol;
+    const double kp_pi = kp * vi.w, kvmu_i = kv * mu_i;
     auto fetch = [&](int j, Rec& r) {
         r.p = ld4(pos + j);
         r.v = ld4(vel + j);
@@ -1080,19 +1081,32 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
         const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : q.p.x - pi.x;
         const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : q.p.y - pi.y;
         const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : q.p.z - pi.z;
-        const double q2 = far_if_self(fma(dz, dz, fma(dy, dy, dx * dx)));
+        // a row's own entry (x_ij = 0 exactly): with one radius and no surface tension every
+        // term below is (finite coefficient) * x_ij once q^2 is kept off zero (1/r = 1e150,
+        // ud / r^2 = 0 * 1e300 = 0), so it adds exact zeros; otherwise it is mapped outside
+        // every support
+        const double q0 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double q2 = (SAMEPV && !SURF) ? fmax(q0, 1.0e-300) : far_if_self(q0);
         const double ir = rsqrt_pair(q2);
         const double r = q2 * ir;
         const int tj = tag_of(q.p.w) & 3;
         // pressure, and viscosity with the harmonic-mean coefficient of the type pair, both
         // along x_ij: one coefficient with their constant factors (kp, kv)
-        const double wp = fmax(fma(-r, C.inv_hp, 1.0), 0.0);
-        const double wv = SAMEPV ? wp : fmax(fma(-r, C.inv_hv, 1.0), 0.0);
-        const double cp = (vi.w + q.v.w) * wp * ir;
-        const double mu = C.mu_uniform ? mu_i : tab[ti * 4 + tj];
         const double ud = fma(q.v.z - vi.z, dz, fma(q.v.y - vi.y, dy, (q.v.x - vi.x) * dx));
-        const double cv = mu * ud * wv * (ir * ir * ir);
-        const double c = fma(kp, cp, kv * cv);
+        const double wp = fmax(fma(-r, C.inv_hp, 1.0), 0.0);
+        double c;
+        if (SAMEPV) {
+            // kp (P_i + P_j) w / r + kv mu ud w / r^3 = (w / r) (kp (P_i + P_j) + kv mu ud / r^2),
+            // kv mu_ij from the table (fewer operations per pair than the sum of the two terms)
+            const double kvmu = C.mu_uniform ? kvmu_i : tab[32 + ti * 4 + tj];
+            c = (wp * ir) * fma(kvmu, ud * (ir * ir), fma(kp, q.v.w, kp_pi));
+        } else {
+            const double wv = fmax(fma(-r, C.inv_hv, 1.0), 0.0);
+            const double cp = (vi.w + q.v.w) * wp * ir;
+            const double mu = C.mu_uniform ? mu_i : tab[ti * 4 + tj];
+            const double cv = mu * ud * wv * (ir * ir * ir);
+            c = fma(kp, cp, kv * cv);
+        }
         acc.px = fma(c, dx, acc.px);
         acc.py = fma(c, dy, acc.py);
         acc.pz = fma(c, dz, acc.pz);
@@ -1143,13 +1157,14 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     const int2 m = cmap[blockIdx.x];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
-    __shared__ double tab[32];
+    __shared__ double tab[48];                 // mu_ij, InteractionRatio, kv mu_ij
     constexpr int S = Frame<K>::S;
     constexpr int NREC = SURF ? 3 : 2;
     __shared__ v2d lds[4][2 * NREC * 64]; // per wavefront: record halves x 64 entries
     if (threadIdx.x < 16) {
         tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
         tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
+        tab[32 + threadIdx.x] = C.visc_c * (-C.dcv) * C.vol * C.muij[threadIdx.x >> 2][threadIdx.x & 3];
     }
     __syncthreads();
     Frame<K> f;
