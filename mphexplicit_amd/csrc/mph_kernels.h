@@ -1093,14 +1093,14 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
         // pressure, and viscosity with the harmonic-mean coefficient of the type pair, both
         // along x_ij: one coefficient with their constant factors (kp, kv)
         const double ud = fma(q.v.z - vi.z, dz, fma(q.v.y - vi.y, dy, (q.v.x - vi.x) * dx));
-        const double wp = fmax(fma(-r, C.inv_hp, 1.0), 0.0);
         double c;
         if (SAMEPV) {
             // kp (P_i + P_j) w / r + kv mu ud w / r^3 = (w / r) (kp (P_i + P_j) + kv mu ud / r^2),
-            // kv mu_ij from the table (fewer operations per pair than the sum of the two terms)
+            // kv mu_ij from the table, and w / r = (1 - r/h) / r = max(1/r - 1/h, 0)
             const double kvmu = C.mu_uniform ? kvmu_i : tab[32 + ti * 4 + tj];
-            c = (wp * ir) * fma(kvmu, ud * (ir * ir), fma(kp, q.v.w, kp_pi));
+            c = fmax(ir - C.inv_hp, 0.0) * fma(kvmu, ud * (ir * ir), fma(kp, q.v.w, kp_pi));
         } else {
+            const double wp = fmax(fma(-r, C.inv_hp, 1.0), 0.0);
             const double wv = fmax(fma(-r, C.inv_hv, 1.0), 0.0);
             const double cp = (vi.w + q.v.w) * wp * ir;
             const double mu = C.mu_uniform ? mu_i : tab[ti * 4 + tj];
