@@ -505,10 +505,10 @@ static int launch_sweep_a(mph_ctx* c, int real, bool count = false)
 static int launch_sweep_b(mph_ctx* c, int real)
 {
     if (c->mixed) {
-        LAUNCH_SK(k_sweep_b_mixed, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream,
-                  c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width,
-                  c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->vnext_out(real), c->force_out(real), c->flags.p,
-                  c->any_flag.p, c->rebuilds.p, real ? c->ncases : 0);
+        LAUNCH_SKC(k_sweep_b_mixed, c->any_surface, c->K, c->same_pv, dim3(c->ncblocks()), dim3(256), 0,
+                   c->stream, c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p,
+                   c->width, c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->vnext_out(real), c->force_out(real),
+                   c->flags.p, c->any_flag.p, c->rebuilds.p, real ? c->ncases : 0);
         return launch_ok("sweep B");
     }
     LAUNCH_SKC(k_sweep_b, c->any_surface, c->K, c->same_pv, dim3(c->ncblocks()), dim3(256), 0, c->stream,

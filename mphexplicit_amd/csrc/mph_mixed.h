@@ -233,7 +233,7 @@ struct AccBF {
     float px, py, pz, vx, vy, vz, ax, ay, az, sx, sy, sz;
 };
 
-template <int SURF, int K, bool WRAP>
+template <int SURF, int K, bool WRAP, bool SAMEPV>
 __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float* tabf, const Frame<K>& f, int ti,
                                                 const double4& pf, float ox, float oy, float oz,
                                                 const v4f& vi, const v4f& ai,
@@ -245,6 +245,8 @@ __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float*
     const float inv_ha = (float)C.inv_ha, inv_hp = (float)C.inv_hp, inv_hv = (float)C.inv_hv;
     const float dcg = (float)C.dcg, cg = (float)C.cg;
     const float mu_i = tabf[ti * 4 + ti];
+    // SAMEPV: one FP32 sum of the pressure and viscosity terms with their constant factors
+    const float kpf = (float)(C.dcp * C.vol), kp_pi = kpf * vi.w, kvmu_i = tabf[32 + ti * 4 + ti];
     auto fetch = [&](int j, RecF& r) {
         const double4 p = pos[j];
         r.p = v4f{offset_f<WRAP>(p.x, pf.x, C.hwid[0], C.dwid[0]), offset_f<WRAP>(p.y, pf.y, C.hwid[1], C.dwid[1]),
@@ -267,16 +269,26 @@ __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float*
         const float ir = __builtin_amdgcn_rsqf(q2);
         const float r = q2 * ir;
         const int tj = __float_as_int(q.p.w);
-        const float cp = (vi.w + q.v.w) * fmaxf(fmaf(-r, inv_hp, 1.f), 0.f) * ir;
-        acc.px = fmaf(cp, dx, acc.px);
-        acc.py = fmaf(cp, dy, acc.py);
-        acc.pz = fmaf(cp, dz, acc.pz);
-        const float mu = C.mu_uniform ? mu_i : tabf[ti * 4 + tj];
         const float ud = fmaf(q.v.z - vi.z, dz, fmaf(q.v.y - vi.y, dy, (q.v.x - vi.x) * dx));
-        const float cv = mu * ud * fmaxf(fmaf(-r, inv_hv, 1.f), 0.f) * (ir * ir * ir);
-        acc.vx = fmaf(cv, dx, acc.vx);
-        acc.vy = fmaf(cv, dy, acc.vy);
-        acc.vz = fmaf(cv, dz, acc.vz);
+        if (SAMEPV) {
+            // (w / r) (kp (P_i + P_j) + kv mu ud / r^2) with w / r = max(1/r - 1/h, 0), as the
+            // FP64 sweep
+            const float kvmu = C.mu_uniform ? kvmu_i : tabf[32 + ti * 4 + tj];
+            const float c = fmaxf(ir - inv_hp, 0.f) * fmaf(kvmu, ud * (ir * ir), fmaf(kpf, q.v.w, kp_pi));
+            acc.px = fmaf(c, dx, acc.px);
+            acc.py = fmaf(c, dy, acc.py);
+            acc.pz = fmaf(c, dz, acc.pz);
+        } else {
+            const float cp = (vi.w + q.v.w) * fmaxf(fmaf(-r, inv_hp, 1.f), 0.f) * ir;
+            acc.px = fmaf(cp, dx, acc.px);
+            acc.py = fmaf(cp, dy, acc.py);
+            acc.pz = fmaf(cp, dz, acc.pz);
+            const float mu = C.mu_uniform ? mu_i : tabf[ti * 4 + tj];
+            const float cv = mu * ud * fmaxf(fmaf(-r, inv_hv, 1.f), 0.f) * (ir * ir * ir);
+            acc.vx = fmaf(cv, dx, acc.vx);
+            acc.vy = fmaf(cv, dy, acc.vy);
+            acc.vz = fmaf(cv, dz, acc.vz);
+        }
         if (SURF) {
             const float rij = C.ratio_one ? 1.f : tabf[16 + ti * 4 + tj];
             const float rji = C.ratio_one ? 1.f : tabf[16 + tj * 4 + ti];
@@ -307,7 +319,7 @@ __device__ __forceinline__ float slot_sum_f(float v)
     return v;
 }
 
-template <int SURF, int K>
+template <int SURF, int K, bool SAMEPV>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 7, 8)))
 k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
                 const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
@@ -328,12 +340,13 @@ k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
     const int2 m = cmap[blockIdx.x];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
-    __shared__ float tabf[32];
+    __shared__ float tabf[48];                 // mu_ij, InteractionRatio, kv mu_ij
     constexpr int NREC = SURF ? 3 : 2;
     __shared__ v4f lds[4][NREC * 64];          // per wavefront: records x 64 entries
     if (threadIdx.x < 16) {
         tabf[threadIdx.x] = (float)C.muij[threadIdx.x >> 2][threadIdx.x & 3];
         tabf[16 + threadIdx.x] = (float)C.ir[threadIdx.x >> 2][threadIdx.x & 3];
+        tabf[32 + threadIdx.x] = (float)(C.visc_c * (-C.dcv) * C.vol * C.muij[threadIdx.x >> 2][threadIdx.x & 3]);
     }
     __syncthreads();
     Frame<K> f;
@@ -351,14 +364,14 @@ k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
         const float ox = offset_f<false>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
         const float oy = offset_f<false>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
         const float oz = offset_f<false>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
-        sweep_b_pairs_f<SURF, K, false>(C, tabf, f, ti, pf, ox, oy, oz, vif, aif, pos, vel, aux, lds[w],
-                                        acc);
+        sweep_b_pairs_f<SURF, K, false, SAMEPV>(C, tabf, f, ti, pf, ox, oy, oz, vif, aif, pos, vel, aux,
+                                                lds[w], acc);
     } else {
         const float ox = offset_f<true>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
         const float oy = offset_f<true>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
         const float oz = offset_f<true>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
-        sweep_b_pairs_f<SURF, K, true>(C, tabf, f, ti, pf, ox, oy, oz, vif, aif, pos, vel, aux, lds[w],
-                                       acc);
+        sweep_b_pairs_f<SURF, K, true, SAMEPV>(C, tabf, f, ti, pf, ox, oy, oz, vif, aif, pos, vel, aux,
+                                               lds[w], acc);
     }
     acc.px = slot_sum_f<K>(acc.px); acc.py = slot_sum_f<K>(acc.py); acc.pz = slot_sum_f<K>(acc.pz);
     acc.vx = slot_sum_f<K>(acc.vx); acc.vy = slot_sum_f<K>(acc.vy); acc.vz = slot_sum_f<K>(acc.vz);
@@ -370,7 +383,7 @@ k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
     const int i = f.i;
     if (count_out[i] == 0)                     // no list (main.cpp:1416): all sums are zero
         acc = AccBF{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    const double cp = C.dcp * C.vol;
+    const double cp = SAMEPV ? 1.0 : C.dcp * C.vol;      // SAMEPV sums carry their factors
     const double cv = C.visc_c * (-C.dcv) * C.vol;
     double fx = (double)acc.px * cp, fy = (double)acc.py * cp, fz = (double)acc.pz * cp;
     if (SURF) {
@@ -379,7 +392,9 @@ k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
         fx += (double)acc.ax * ca; fy += (double)acc.ay * ca; fz += (double)acc.az * ca;
         fx += (double)acc.sx * cs; fy += (double)acc.sy * cs; fz += (double)acc.sz * cs;
     }
-    fx += (double)acc.vx * cv; fy += (double)acc.vy * cv; fz += (double)acc.vz * cv;
+    if (!SAMEPV) {
+        fx += (double)acc.vx * cv; fy += (double)acc.vy * cv; fz += (double)acc.vz * cv;
+    }
     double4 vn = make_double4(vi.x, vi.y, vi.z, 0.0);
     if (ti < 2) {
         const double ms = C.mass[ti];
