@@ -329,6 +329,13 @@ def small_dam_break_3d(seed=12) -> Deck:
                         name="small_dambreak3d")
 
 
+def wide_dam_break_3d(seed=15) -> Deck:
+    """Small 3D dam break whose every width exceeds eight list cut-offs (as the 1M deck's do):
+    10x30x10 fluid in a 30x30x30 walled box, y periodic."""
+    return dam_break_3d(fluid=(10, 30, 10), box=(30, 30, 30), jitter=0.01, seed=seed, steps=100,
+                        name="wide_dambreak3d")
+
+
 def drift_dam_break_3d(seed=14) -> Deck:
     """Small 3D dam break for 1000-step runs: 8x11x8 fluid in a 14x11x12 walled box."""
     return dam_break_3d(fluid=(8, 11, 8), box=(14, 11, 12), jitter=0.01, seed=seed, steps=1000,

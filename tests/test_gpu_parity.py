@@ -22,6 +22,8 @@ DECKS = {
     "movingwall2d": decks.small_moving_wall_2d,
     "dambreak3d": decks.small_dam_break_3d,
     "twofluid3d": decks.small_two_fluid_3d,
+    # every width above eight cut-offs: the list build's bounding-box test is on in 3D
+    "wide3d": decks.wide_dam_break_3d,
 }
 
 COMPARED = ("x", "v", "force", "dens_a", "vol_strain", "divergence", "press_p", "press_a", "gcenter")
@@ -105,7 +107,7 @@ def test_first_step_parity(solver_cls, name):
     assert s.time() == orc.time
 
 
-@pytest.mark.parametrize("name", ["droplet2d", "movingwall2d", "bridge2d", "squaredrop2d", "twofluid3d"])
+@pytest.mark.parametrize("name", ["droplet2d", "movingwall2d", "bridge2d", "squaredrop2d", "twofluid3d", "wide3d"])
 def test_multi_step_parity(solver_cls, name):
     print("case", name, flush=True)
     d = DECKS[name]()
@@ -188,7 +190,7 @@ def test_golden_fixtures(solver_cls, name):
 
 
 @pytest.mark.parametrize("K", ["2", "4", "8"])
-@pytest.mark.parametrize("name", ["twofluid3d", "movingwall2d"])
+@pytest.mark.parametrize("name", ["twofluid3d", "movingwall2d", "wide3d"])
 def test_every_cluster_size(solver_cls, name, K, monkeypatch):
     # each tile shape the autotuner can pick gives the reference's lists and fields
     monkeypatch.setenv("MPH_TUNE", K)
