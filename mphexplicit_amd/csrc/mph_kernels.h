@@ -726,9 +726,10 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
 }
 
 // -------------------------------------------------------------------------------------
-// Cluster frame of a wavefront in the sweep kernels: which cluster, which row/slot, and
-// the software-pipelined walk over the union list (indices two steps ahead, partner data
-// one step ahead in alternating register sets).
+// Cluster frame of a wavefront in the sweep kernels: which cluster, which row/slot.  The
+// pair sweeps walk the union list through LDS blocks (walk_union_lds); the virial, run at
+// output steps only, uses the register-pipelined walk_union (indices two steps ahead,
+// partner data one step ahead in alternating register sets).
 
 template <int K>
 struct Frame {
