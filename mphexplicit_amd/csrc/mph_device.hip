@@ -107,11 +107,13 @@ struct mph_ctx {
     int* h_scalar = nullptr;   // pinned host word for small readbacks
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_src = nullptr;
+    hipEvent_t ev_flag = nullptr;     // the eager step's flag readback has landed
 
     ~mph_ctx()
     {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (graph_src) (void)hipGraphDestroy(graph_src);
+        if (ev_flag) (void)hipEventDestroy(ev_flag);
         if (h_scalar) (void)hipHostFree(h_scalar);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -476,15 +478,15 @@ static int launch_sweep_a(mph_ctx* c, int real, bool count = false)
     if (c->mixed) {
         LAUNCH_SK(k_sweep_a_mixed, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream,
                   c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width,
-                  c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p, c->wallc.p,
-                  real ? c->ncases : 0, real ? c->pairs.p : c->pairs_scratch.p);
+                  c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p,
+                  real ? c->pairs.p : c->pairs_scratch.p);
         return launch_ok("sweep A");
     }
     const int mode = (c->same_ap ? 1 : 0) | (c->ratio_one ? 2 : 0);
 #define SWEEP_A_ARGS                                                                                        \
     dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p,  \
         c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p,          \
-        c->vstrain.p, c->divp.p, c->wallc.p, real ? c->ncases : 0, real ? c->pairs.p : c->pairs_scratch.p
+        c->vstrain.p, c->divp.p, real ? c->pairs.p : c->pairs_scratch.p
     const bool cnt = count && real;
 #define SWEEP_A_MODE(M)                                                                            \
     do {                                                                                           \
@@ -508,14 +510,14 @@ static int launch_sweep_b(mph_ctx* c, int real)
         LAUNCH_SKC(k_sweep_b_mixed, c->any_surface, c->K, c->same_pv, dim3(c->ncblocks()), dim3(256), 0,
                    c->stream, c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p,
                    c->width, c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->vnext_out(real), c->force_out(real),
-                   c->flags.p, c->any_flag.p, c->rebuilds.p, real ? c->ncases : 0);
+                   c->flags.p, c->any_flag.p, c->rebuilds.p, c->wallc.p, real ? c->ncases : 0);
         return launch_ok("sweep B");
     }
     LAUNCH_SKC(k_sweep_b, c->any_surface, c->K, c->same_pv, dim3(c->ncblocks()), dim3(256), 0, c->stream,
                c->d_cases.p,
               c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p,
               c->pos.p, c->vel.p, c->aux.p, c->vnext_out(real), c->force_out(real), c->flags.p,
-              c->any_flag.p, c->rebuilds.p, real ? c->ncases : 0);
+              c->any_flag.p, c->rebuilds.p, c->wallc.p, real ? c->ncases : 0);
     return launch_ok("sweep B");
 }
 
@@ -546,25 +548,24 @@ static int read_int(mph_ctx* c, const int* dptr, int* out)
 // The eager step decides the rebuild on the host, as the reference's loop does
 // (main.cpp:386-395): the Verlet flag of the move is read back and the 25 rebuild launches
 // are issued only when it is set.  Launched unconditionally they return at once without a
-// rebuild, but each costs about 4.4 us on the GPU (0.11 ms per step at 1.27M particles); the
-// readback costs one short pause.  Graph replay keeps the device-side decision.
+// rebuild, but each costs about 4.4 us on the GPU (0.11 ms per step at 1.27M particles).
+// Sweep A is queued before the flag is known: it is right unless this step rebuilds the
+// lists, and then it is queued again after the rebuild (it has no side effect beyond its own
+// outputs), so the GPU does not wait for the host.  Graph replay keeps the device-side
+// decision.
 static int launch_step_host(mph_ctx* c)
 {
     int rc = launch_move(c);
-    int flag = 0;
-    if (!rc) {
-        // a short wait: poll the stream rather than block in the runtime
-        HIP_TRY(hipMemcpyAsync(c->h_scalar, c->any_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        hipError_t q;
-        while ((q = hipStreamQuery(c->stream)) == hipErrorNotReady) {
-        }
-        if (q != hipSuccess) return fail(MPH_ERR_DEVICE, "step: %s", hipGetErrorString(q));
-        flag = *c->h_scalar;
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->h_scalar, c->any_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipEventRecord(c->ev_flag, c->stream));
+    if ((rc = launch_sweep_a(c, 1))) return rc;
+    hipError_t q;
+    while ((q = hipEventQuery(c->ev_flag)) == hipErrorNotReady) {
     }
-    if (!rc && flag) rc = launch_rebuild(c, 0);
-    if (!rc) rc = launch_sweep_a(c, 1);
-    if (!rc) rc = launch_sweep_b(c, 1);
-    return rc;
+    if (q != hipSuccess) return fail(MPH_ERR_DEVICE, "step: %s", hipGetErrorString(q));
+    if (*c->h_scalar && ((rc = launch_rebuild(c, 0)) || (rc = launch_sweep_a(c, 1)))) return rc;
+    return launch_sweep_b(c, 1);
 }
 
 static int check_capacity(mph_ctx* c)
@@ -734,6 +735,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         return fail(MPH_ERR_DEVICE, "no GPU available");
     HIP_TRY(hipGetDevice(&c->device));
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_flag, hipEventDisableTiming));
     HIP_TRY(hipHostMalloc((void**)&c->h_scalar, sizeof(long long) * 4, hipHostMallocDefault));
     c->ncases = ncases;
     c->cases.resize(ncases);

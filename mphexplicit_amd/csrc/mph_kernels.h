@@ -974,16 +974,9 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int* __restrict__ count_out,
           const double4* __restrict__ pos, double4* vel, double4* __restrict__ aux,
           double* __restrict__ dens_a, double* __restrict__ vstrain, double* __restrict__ divp,
-          double* __restrict__ wallc, int ncases, unsigned long long* __restrict__ pair_count)
+          unsigned long long* __restrict__ pair_count)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        // wall centres advance once per step (main.cpp:2048-2052); the move kernel of
-        // this step has already used the old value
-        for (int c = 0; c < ncases; ++c)
-            for (int t = 2; t < kTypes; ++t)
-                for (int d = 0; d < 3; ++d)
-                    wallc[(c * kTypes + t) * 3 + d] += cases[c].wvel[t][d] * cases[c].dt;
-    }
+    // no side effect beyond its outputs: the eager step may run it once more after a rebuild
     const int2 m = cmap[blockIdx.x];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
@@ -1147,12 +1140,17 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const double4* __restrict__ pos, const double4* __restrict__ vel,
           const double4* __restrict__ aux, double4* __restrict__ vnext,
           double4* __restrict__ force, int* __restrict__ flags,
-          int* __restrict__ any_flag, long long* __restrict__ rebuilds, int ncases)
+          int* __restrict__ any_flag, long long* __restrict__ rebuilds, double* __restrict__ wallc, int ncases)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0 && ncases > 0) {
         for (int c = 0; c < ncases; ++c) {
             if (flags[c]) rebuilds[c] += 1;
             flags[c] = 0;
+            // wall centres advance once per step (main.cpp:2048-2052); the move kernel of
+            // this step has already used the old value
+            for (int t = 2; t < kTypes; ++t)
+                for (int d = 0; d < 3; ++d)
+                    wallc[(c * kTypes + t) * 3 + d] += cases[c].wvel[t][d] * cases[c].dt;
         }
         *any_flag = 0;
     }
