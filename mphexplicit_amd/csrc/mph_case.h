@@ -20,7 +20,7 @@ struct CaseConst {
     int rank_base;          // offset of this case in the cell rank tables
     int mu_uniform;         // every type has the same shear viscosity
     int cl_base, cl_cap;    // this case's cluster slots [cl_base, cl_base + cl_cap)
-    int pad0;
+    int img_axes;           // bit d: images near a cluster's first member are exact on axis d
     double dmin[3], dmax[3], dwid[3], hwid[3];   // domain min, max, width, 0.5*width
     double guard[3];        // distance from a face beyond which no list partner wraps
     double cw[3];                       // cell widths

@@ -319,6 +319,12 @@ static int derive_case(const mph_params& p, int begin, int end, int nfluid, int 
     // moved less than margin/2 since (main.cpp:1102)
     const double reach = maxr + 2.5 * margin;
     for (int d = 0; d < 3; ++d) k.guard[d] = k.hwid[d] > 2.0 * reach ? reach : HUGE_VAL;
+    // the members of a cluster shared a cell at the last rebuild and have moved less than
+    // margin/2 since, and an interacting partner is within maxr of its row: a cell plus reach
+    // below half the width makes the image near the first member the minimum image
+    k.img_axes = 0;
+    for (int d = 0; d < 3; ++d)
+        if (k.hwid[d] > k.cw[d] + reach) k.img_axes |= 1 << d;
     k.hmax2 = maxr * maxr;
     k.shift2 = 0.5 * margin * 0.5 * margin;                             // main.cpp:1102
     return MPH_OK;

@@ -86,6 +86,18 @@ __device__ __forceinline__ double rel_wrap(double xj, double xi, double hw, doub
     return (a - s) - hw;
 }
 
+// The image x - W, x or x + W of a coordinate nearest to a reference point xf (a cluster's
+// first member).  On an axis where a cell plus the reach of a list pair stays below W/2,
+// the images of a row and of its interacting partners near the same xf differ by their
+// minimum-image difference, and a row's own entry gives exactly zero: the periodic path of
+// the sweeps takes the images once per union entry when it is fetched (shared by the K rows)
+// and once per row, and then uses the plain difference.
+__device__ __forceinline__ double image_near(double x, double xf, double hw, double w)
+{
+    const double a = x - xf + hw;
+    return a < 0.0 ? x + w : (a >= w ? x - w : x);
+}
+
 // For a particle far enough from the periodic faces that no list partner wraps, the pair
 // sweeps use the plain difference x_j - x_i: the exact minimum image, which differs from
 // the reference's Mod form only by the rounding of the +-W/2 round trip.  Discrete
@@ -806,8 +818,15 @@ struct Rec {
 // far_if_self in the evaluators, so the loop over a block is straight-line code.
 // NREC = 3 adds the gravity-centre record (surface tension).  PL3: records without the tag
 // and P, in three planes (x, y | vx, vy | z, vz): 48 bytes of LDS per entry instead of 64.
-template <int K, int NREC, class Fetch, class Eval, bool PL3 = false>
-__device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restrict__ lds, Fetch fetch, Eval eval)
+// Applied to each fetched record just before it is stored to LDS (its loads have landed by
+// then): the periodic path's images near the cluster's first member, or nothing.
+struct NoPrep {
+    __device__ void operator()(Rec&) const {}
+};
+
+template <int K, int NREC, class Fetch, class Eval, bool PL3 = false, class Prep = NoPrep>
+__device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restrict__ lds, Fetch fetch, Eval eval,
+                                               Prep prep = Prep())
 {
     // LDS slice: 2*NREC planes of 64 16-byte halves (x,y | z,w of each record), so the
     // 16 or 32 distinct entries one read touches are contiguous and free of bank conflicts
@@ -828,6 +847,7 @@ __device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restric
         // before the reads below; the barriers keep the compiler from moving LDS
         // accesses across the block boundary
         __builtin_amdgcn_wave_barrier();
+        prep(r);
         if (PL3) {
             lds[0 * 64 + lane] = r.p.xy;
             lds[1 * 64 + lane] = r.v.xy;
@@ -902,6 +922,43 @@ __device__ __forceinline__ bool away(const CaseConst& C, const double4& p)
     return away_from_faces(C, p);
 }
 
+// Periodic path of a wavefront: 0 if no row is near a face (plain differences), 1 if the
+// axes its rows are near all take images near the first member, 2 otherwise (the exact
+// periodic difference per row and pair).  axes: the axes that need images (uniform).
+__device__ __forceinline__ int wrap_path(const CaseConst& C, const double4& p, int& axes)
+{
+    const bool nx = !(p.x - C.dmin[0] >= C.guard[0] && C.dmax[0] - p.x >= C.guard[0]);
+    const bool ny = !(p.y - C.dmin[1] >= C.guard[1] && C.dmax[1] - p.y >= C.guard[1]);
+    const bool nz = !(p.z - C.dmin[2] >= C.guard[2] && C.dmax[2] - p.z >= C.guard[2]);
+    axes = (__ballot(nx) != 0ull ? 1 : 0) | (__ballot(ny) != 0ull ? 2 : 0) | (__ballot(nz) != 0ull ? 4 : 0);
+    return axes == 0 ? 0 : ((axes & ~C.img_axes) == 0 ? 1 : 2);
+}
+
+// a position's images near xf on the given axes
+__device__ __forceinline__ void image_axes(const CaseConst& C, int axes, const double4& xf, double& x, double& y,
+                                           double& z)
+{
+    if (axes & 1) x = image_near(x, xf.x, C.hwid[0], C.dwid[0]);
+    if (axes & 2) y = image_near(y, xf.y, C.hwid[1], C.dwid[1]);
+    if (axes & 4) z = image_near(z, xf.z, C.hwid[2], C.dwid[2]);
+}
+
+template <bool ON>
+struct ImagePrep {
+    const CaseConst& C;
+    const double4& xf;
+    int axes;
+    __device__ void operator()(Rec& r) const
+    {
+        if (!ON) return;
+        double x = r.p.x, y = r.p.y, z = r.p.z;
+        image_axes(C, axes, xf, x, y, z);
+        r.p.x = x;
+        r.p.y = y;
+        r.p.z = z;
+    }
+};
+
 // -------------------------------------------------------------------------------------
 // sweep A: number densities, gravity centre, volumetric strain, divergence
 // (main.cpp:1665-1727, 1831-1896) with the pointwise coefficients and pressures of
@@ -919,11 +976,12 @@ struct AccA {
     int np;
 };
 
-template <int SURF, int K, bool WRAP, bool COUNT, int MODE>
+// WRAP: the periodic path of wrap_path (pi holds the row's images on path 1)
+template <int SURF, int K, int WRAP, bool COUNT, int MODE>
 __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, double vix, double viy, double viz,
                                               const double4* __restrict__ pos, const double4* vel,
-                                              v2d* lds, AccA& acc)
+                                              v2d* lds, AccA& acc, const double4& pf, int axes)
 {
     constexpr bool SAMEAP = (MODE & 1) != 0;
     constexpr bool R1 = (MODE & 2) != 0 && !SURF;     // every InteractionRatio is 1: no tag needed
@@ -934,9 +992,9 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         r.v = v4d{u.x, u.y, u.z, 0.0};
     };
     auto eval = [&](const Rec& q) {
-        const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : q.p.x - pi.x;
-        const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : q.p.y - pi.y;
-        const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : q.p.z - pi.z;
+        const double dx = WRAP == 2 ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : q.p.x - pi.x;
+        const double dy = WRAP == 2 ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : q.p.y - pi.y;
+        const double dz = WRAP == 2 ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : q.p.z - pi.z;
         const double q2 = far_if_self(fma(dz, dz, fma(dy, dy, dx * dx)));
         if (COUNT) acc.np += q2 <= C.hmax2;
         const double ir = rsqrt_pair(q2);
@@ -956,18 +1014,19 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         const double ud = fma(q.v.z - viz, dz, fma(q.v.y - viy, dy, (q.v.x - vix) * dx));
         acc.ds = fma(ud, omp * ir, acc.ds);
     };
-    walk_union_lds<K, 2, decltype(fetch), decltype(eval), R1>(f, lds, fetch, eval);
+    walk_union_lds<K, 2, decltype(fetch), decltype(eval), R1>(f, lds, fetch, eval, ImagePrep<WRAP == 1>{C, pf, axes});
 }
 
-// Without surface tension both sweeps run at six wavefronts per SIMD (at most 80 VGPRs;
-// sweep A needs 78, sweep B spills a few bytes outside the pair loop), measured faster than
-// five; sweep A with three-plane records (MODE bit 1) runs at seven (72 VGPRs).
+// Without surface tension both sweeps run at six wavefronts per SIMD (at most 80 VGPRs, a
+// few bytes spilled outside the pair loop), measured faster than five and seven, also with
+// the three-plane records of MODE bit 1; with surface tension sweep A is held to five (92
+// VGPRs).
 // COUNT: also count the interacting pairs (r <= largest radius) into pair_count; only the
 // timed steps that report pair rates use it.  MODE bit 0: the attractive and pressure
 // kernels have the same radius (one weight per pair); bit 1: every InteractionRatio is 1, so
 // without surface tension the partner records need no tag (three LDS planes)
 template <int SURF, int K, bool COUNT, int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : ((MODE & 2) ? 7 : 6), 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 5 : 6, 8)))
 k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
           const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
@@ -995,13 +1054,20 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     const int ti = tag_of(pi.w) & 3;
     const double vix = vel[f.i].x, viy = vel[f.i].y, viz = vel[f.i].z;
     AccA acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
-    // a wavefront takes the periodic path if any of its rows needs it (uniform branch)
-    if (__ballot(!away(C, pi)) == 0ull)
-        sweep_a_pairs<SURF, K, false, COUNT, MODE>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w],
-                                                     acc);
-    else
-        sweep_a_pairs<SURF, K, true, COUNT, MODE>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w],
-                                                    acc);
+    // a wavefront takes a periodic path if any of its rows needs it (uniform branch)
+    int axes;
+    const int path = wrap_path(C, pi, axes);
+    if (path == 0) {
+        sweep_a_pairs<SURF, K, 0, COUNT, MODE>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], acc, pi, 0);
+    } else if (path == 1) {
+        const double4 pf = pos[__builtin_amdgcn_readfirstlane(f.first)];   // uniform
+        double4 pim = pi;
+        image_axes(C, axes, pf, pim.x, pim.y, pim.z);
+        sweep_a_pairs<SURF, K, 1, COUNT, MODE>(C, tab, f, ti, pim, vix, viy, viz, pos, vel, lds[w], acc, pf,
+                                               axes);
+    } else {
+        sweep_a_pairs<SURF, K, 2, COUNT, MODE>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], acc, pi, 0);
+    }
     acc.na = slot_sum<K>(acc.na);
     acc.ws = slot_sum<K>(acc.ws);
     acc.ds = slot_sum<K>(acc.ds);
@@ -1057,11 +1123,12 @@ struct AccB {
     double sx, sy, sz;      // diffuse interface
 };
 
-template <int SURF, int K, bool WRAP, bool SAMEPV>
+template <int SURF, int K, int WRAP, bool SAMEPV>
 __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* tab, const Frame<K>& f, int ti,
                                               const double4& pi, const double4& vi, const double4& ai,
                                               const double4* __restrict__ pos, const double4* __restrict__ vel,
-                                              const double4* __restrict__ aux, v2d* lds, AccB& acc)
+                                              const double4* __restrict__ aux, v2d* lds, AccB& acc,
+                                              const double4& pf, int axes)
 {
     const double mu_i = tab[ti * 4 + ti];
     const double kp = C.dcp * C.vol, kv = C.visc_c * (-C.dcv) * C.vol;
@@ -1072,9 +1139,9 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
         if (SURF) r.a = ld4(aux + j);
     };
     auto eval = [&](const Rec& q) {
-        const double dx = WRAP ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : q.p.x - pi.x;
-        const double dy = WRAP ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : q.p.y - pi.y;
-        const double dz = WRAP ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : q.p.z - pi.z;
+        const double dx = WRAP == 2 ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : q.p.x - pi.x;
+        const double dy = WRAP == 2 ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : q.p.y - pi.y;
+        const double dz = WRAP == 2 ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : q.p.z - pi.z;
         // a row's own entry (x_ij = 0 exactly): with one radius and no surface tension every
         // term below is (finite coefficient) * x_ij once q^2 is kept off zero (1/r = 1e150,
         // ud / r^2 = 0 * 1e300 = 0), so it adds exact zeros; otherwise it is mapped outside
@@ -1125,7 +1192,8 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
             acc.sz = fma(-om, fma(qd, dz, w * qz), acc.sz);
         }
     };
-    walk_union_lds<K, SURF ? 3 : 2>(f, lds, fetch, eval);
+    walk_union_lds<K, SURF ? 3 : 2, decltype(fetch), decltype(eval), false>(f, lds, fetch, eval,
+                                                                          ImagePrep<WRAP == 1>{C, pf, axes});
 }
 
 // SAMEPV: the pressure and viscosity kernels have the same radius (one weight per pair).
@@ -1175,10 +1243,18 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     const int ti = tag_of(pi.w) & 3;
     const double4 ai = SURF ? aux[f.i] : make_double4(0.0, 0.0, 0.0, 0.0);
     AccB acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (__ballot(!away(C, pi)) == 0ull)
-        sweep_b_pairs<SURF, K, false, SAMEPV>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], acc);
-    else
-        sweep_b_pairs<SURF, K, true, SAMEPV>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], acc);
+    int axes;
+    const int path = wrap_path(C, pi, axes);
+    if (path == 0) {
+        sweep_b_pairs<SURF, K, 0, SAMEPV>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], acc, pi, 0);
+    } else if (path == 1) {
+        const double4 pf = pos[__builtin_amdgcn_readfirstlane(f.first)];   // uniform
+        double4 pim = pi;
+        image_axes(C, axes, pf, pim.x, pim.y, pim.z);
+        sweep_b_pairs<SURF, K, 1, SAMEPV>(C, tab, f, ti, pim, vi, ai, pos, vel, aux, lds[w], acc, pf, axes);
+    } else {
+        sweep_b_pairs<SURF, K, 2, SAMEPV>(C, tab, f, ti, pi, vi, ai, pos, vel, aux, lds[w], acc, pi, 0);
+    }
     acc.px = slot_sum<K>(acc.px); acc.py = slot_sum<K>(acc.py); acc.pz = slot_sum<K>(acc.pz);
     if (SURF) {
         acc.ax = slot_sum<K>(acc.ax); acc.ay = slot_sum<K>(acc.ay); acc.az = slot_sum<K>(acc.az);

@@ -30,11 +30,14 @@ __device__ __forceinline__ float fold_f(float d, float hw, float w)
     return d >= hw ? d - w : (d < -hw ? d + w : d);
 }
 
-// FP32 offset of x from the reference point xf on one axis
-template <bool WRAP>
-__device__ __forceinline__ float offset_f(double x, double xf, double hw, double w)
+// FP32 offset of x from the reference point xf on one axis: the plain difference (WRAP 0),
+// the image near xf on the axes that need images (WRAP 1, `on`) or on every axis (WRAP 2).
+// On WRAP 1 those axes take exact images near xf (wrap_path), so the offsets of a row and
+// its partners differ by the minimum image and need no fold per pair.
+template <int WRAP>
+__device__ __forceinline__ float offset_f(double x, double xf, double hw, double w, bool on = true)
 {
-    return (float)(WRAP ? rel_wrap(x, xf, hw, w) : x - xf);
+    return (float)((WRAP == 2 || (WRAP == 1 && on)) ? rel_wrap(x, xf, hw, w) : x - xf);
 }
 
 // FP32 counterparts of kFar and far_if_self: squares of the far offset stay finite in FP32
@@ -90,27 +93,28 @@ __device__ __forceinline__ void walk_union_lds_f(const Frame<K>& f, v4f* __restr
 // sweep A, mixed: same sums and epilogue as k_sweep_a (main.cpp:1665-1727, 1831-1906,
 // 1729-1740)
 
-template <int SURF, int K, bool WRAP>
+template <int SURF, int K, int WRAP>
 __device__ __forceinline__ void sweep_a_pairs_f(const CaseConst& C, const float* tabf, const Frame<K>& f, int ti,
                                                 const double4& pf, float ox, float oy, float oz,
                                                 float vix, float viy, float viz,
                                                 const double4* __restrict__ pos, const double4* vel,
                                                 v4f* lds, AccA& acc, float& gxf, float& gyf,
-                                                float& gzf)
+                                                float& gzf, int axes)
 {
     const float hwx = (float)C.hwid[0], hwy = (float)C.hwid[1], hwz = (float)C.hwid[2];
     const float wx = (float)C.dwid[0], wy = (float)C.dwid[1], wz = (float)C.dwid[2];
     const float inv_ha = (float)C.inv_ha, inv_hp = (float)C.inv_hp, hmax2 = (float)C.hmax2;
     auto fetch = [&](int j, RecF& r) {
         const double4 p = pos[j];
-        r.p = v4f{offset_f<WRAP>(p.x, pf.x, C.hwid[0], C.dwid[0]), offset_f<WRAP>(p.y, pf.y, C.hwid[1], C.dwid[1]),
-                  offset_f<WRAP>(p.z, pf.z, C.hwid[2], C.dwid[2]), __int_as_float(tag_of(p.w) & 3)};
+        r.p = v4f{offset_f<WRAP>(p.x, pf.x, C.hwid[0], C.dwid[0], axes & 1),
+                  offset_f<WRAP>(p.y, pf.y, C.hwid[1], C.dwid[1], axes & 2),
+                  offset_f<WRAP>(p.z, pf.z, C.hwid[2], C.dwid[2], axes & 4), __int_as_float(tag_of(p.w) & 3)};
         // P (vel.w) of partners is being written by this kernel: read x, y, z only
         r.v = v4f{(float)vel[j].x, (float)vel[j].y, (float)vel[j].z, 0.f};
     };
     auto eval = [&](const RecF& q) {
         float dx = q.p.x - ox, dy = q.p.y - oy, dz = q.p.z - oz;
-        if (WRAP) {
+        if (WRAP == 2) {
             dx = fold_f(dx, hwx, wx);
             dy = fold_f(dy, hwy, wy);
             dz = fold_f(dz, hwz, wz);
@@ -169,19 +173,20 @@ k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
     const float vix = (float)vel[f.i].x, viy = (float)vel[f.i].y, viz = (float)vel[f.i].z;
     AccA acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
     float gxf = 0.f, gyf = 0.f, gzf = 0.f;
-    if (__ballot(!away(C, pi)) == 0ull) {
-        const float ox = offset_f<false>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
-        const float oy = offset_f<false>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
-        const float oz = offset_f<false>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
-        sweep_a_pairs_f<SURF, K, false>(C, tabf, f, ti, pf, ox, oy, oz, vix, viy, viz, pos, vel, lds[w],
-                                        acc, gxf, gyf, gzf);
-    } else {
-        const float ox = offset_f<true>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
-        const float oy = offset_f<true>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
-        const float oz = offset_f<true>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
-        sweep_a_pairs_f<SURF, K, true>(C, tabf, f, ti, pf, ox, oy, oz, vix, viy, viz, pos, vel, lds[w],
-                                       acc, gxf, gyf, gzf);
-    }
+    int axes;
+    const int path = wrap_path(C, pi, axes);
+#define MPH_SWEEP_A_F(P)                                                                              \
+    sweep_a_pairs_f<SURF, K, P>(C, tabf, f, ti, pf, offset_f<P>(pi.x, pf.x, C.hwid[0], C.dwid[0], axes & 1), \
+                                offset_f<P>(pi.y, pf.y, C.hwid[1], C.dwid[1], axes & 2),              \
+                                offset_f<P>(pi.z, pf.z, C.hwid[2], C.dwid[2], axes & 4), vix, viy, viz, \
+                                pos, vel, lds[w], acc, gxf, gyf, gzf, axes)
+    if (path == 0)
+        MPH_SWEEP_A_F(0);
+    else if (path == 1)
+        MPH_SWEEP_A_F(1);
+    else
+        MPH_SWEEP_A_F(2);
+#undef MPH_SWEEP_A_F
     acc.na = slot_sum<K>(acc.na);
     acc.ws = slot_sum<K>(acc.ws);
     acc.ds = slot_sum<K>(acc.ds);
@@ -226,12 +231,13 @@ struct AccBF {
     float px, py, pz, vx, vy, vz, ax, ay, az, sx, sy, sz;
 };
 
-template <int SURF, int K, bool WRAP, bool SAMEPV>
+template <int SURF, int K, int WRAP, bool SAMEPV>
 __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float* tabf, const Frame<K>& f, int ti,
                                                 const double4& pf, float ox, float oy, float oz,
                                                 const v4f& vi, const v4f& ai,
                                                 const double4* __restrict__ pos, const double4* __restrict__ vel,
-                                                const double4* __restrict__ aux, v4f* lds, AccBF& acc)
+                                                const double4* __restrict__ aux, v4f* lds, AccBF& acc,
+                                                int axes)
 {
     const float hwx = (float)C.hwid[0], hwy = (float)C.hwid[1], hwz = (float)C.hwid[2];
     const float wx = (float)C.dwid[0], wy = (float)C.dwid[1], wz = (float)C.dwid[2];
@@ -242,8 +248,9 @@ __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float*
     const float kpf = (float)(C.dcp * C.vol), kp_pi = kpf * vi.w, kvmu_i = tabf[32 + ti * 4 + ti];
     auto fetch = [&](int j, RecF& r) {
         const double4 p = pos[j];
-        r.p = v4f{offset_f<WRAP>(p.x, pf.x, C.hwid[0], C.dwid[0]), offset_f<WRAP>(p.y, pf.y, C.hwid[1], C.dwid[1]),
-                  offset_f<WRAP>(p.z, pf.z, C.hwid[2], C.dwid[2]), __int_as_float(tag_of(p.w) & 3)};
+        r.p = v4f{offset_f<WRAP>(p.x, pf.x, C.hwid[0], C.dwid[0], axes & 1),
+                  offset_f<WRAP>(p.y, pf.y, C.hwid[1], C.dwid[1], axes & 2),
+                  offset_f<WRAP>(p.z, pf.z, C.hwid[2], C.dwid[2], axes & 4), __int_as_float(tag_of(p.w) & 3)};
         const double4 u = vel[j];
         r.v = v4f{(float)u.x, (float)u.y, (float)u.z, (float)u.w};
         if (SURF) {
@@ -253,7 +260,7 @@ __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float*
     };
     auto eval = [&](const RecF& q) {
         float dx = q.p.x - ox, dy = q.p.y - oy, dz = q.p.z - oz;
-        if (WRAP) {
+        if (WRAP == 2) {
             dx = fold_f(dx, hwx, wx);
             dy = fold_f(dy, hwy, wy);
             dz = fold_f(dz, hwz, wz);
@@ -358,19 +365,20 @@ k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
     const v4f vif = v4f{(float)vi.x, (float)vi.y, (float)vi.z, (float)vi.w};
     const v4f aif = v4f{(float)ai.x, (float)ai.y, (float)ai.z, (float)ai.w};
     AccBF acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (__ballot(!away(C, pi)) == 0ull) {
-        const float ox = offset_f<false>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
-        const float oy = offset_f<false>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
-        const float oz = offset_f<false>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
-        sweep_b_pairs_f<SURF, K, false, SAMEPV>(C, tabf, f, ti, pf, ox, oy, oz, vif, aif, pos, vel, aux,
-                                                lds[w], acc);
-    } else {
-        const float ox = offset_f<true>(pi.x, pf.x, C.hwid[0], C.dwid[0]);
-        const float oy = offset_f<true>(pi.y, pf.y, C.hwid[1], C.dwid[1]);
-        const float oz = offset_f<true>(pi.z, pf.z, C.hwid[2], C.dwid[2]);
-        sweep_b_pairs_f<SURF, K, true, SAMEPV>(C, tabf, f, ti, pf, ox, oy, oz, vif, aif, pos, vel, aux,
-                                               lds[w], acc);
-    }
+    int axes;
+    const int path = wrap_path(C, pi, axes);
+#define MPH_SWEEP_B_F(P)                                                                              \
+    sweep_b_pairs_f<SURF, K, P, SAMEPV>(C, tabf, f, ti, pf, offset_f<P>(pi.x, pf.x, C.hwid[0], C.dwid[0], axes & 1), \
+                                        offset_f<P>(pi.y, pf.y, C.hwid[1], C.dwid[1], axes & 2),      \
+                                        offset_f<P>(pi.z, pf.z, C.hwid[2], C.dwid[2], axes & 4), vif, aif, \
+                                        pos, vel, aux, lds[w], acc, axes)
+    if (path == 0)
+        MPH_SWEEP_B_F(0);
+    else if (path == 1)
+        MPH_SWEEP_B_F(1);
+    else
+        MPH_SWEEP_B_F(2);
+#undef MPH_SWEEP_B_F
     acc.px = slot_sum_f<K>(acc.px); acc.py = slot_sum_f<K>(acc.py); acc.pz = slot_sum_f<K>(acc.pz);
     acc.vx = slot_sum_f<K>(acc.vx); acc.vy = slot_sum_f<K>(acc.vy); acc.vz = slot_sum_f<K>(acc.vz);
     if (SURF) {
