@@ -986,7 +986,13 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
     constexpr bool SAMEAP = (MODE & 1) != 0;
     constexpr bool R1 = (MODE & 2) != 0 && !SURF;     // every InteractionRatio is 1: no tag needed
     auto fetch = [&](int j, Rec& r) {
-        r.p = ld4(pos + j);
+        if (R1) {
+            // no tag needed: x, y, z only (two fewer registers per record in flight)
+            const double4 p = pos[j];
+            r.p = v4d{p.x, p.y, p.z, 0.0};
+        } else {
+            r.p = ld4(pos + j);
+        }
         // P (vel.w) of partners is being written by this kernel: read x, y, z only
         const double4 u = vel[j];
         r.v = v4d{u.x, u.y, u.z, 0.0};
@@ -1132,7 +1138,7 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
 {
     const double mu_i = tab[ti * 4 + ti];
     const double kp = C.dcp * C.vol, kv = C.visc_c * (-C.dcv) * C.vol;
-    const double kp_pi = kp * vi.w, kvmu_i = kv * mu_i;
+    const double kp_pi = kp * vi.w;
     auto fetch = [&](int j, Rec& r) {
         r.p = ld4(pos + j);
         r.v = ld4(vel + j);
@@ -1158,7 +1164,8 @@ __device__ __forceinline__ void sweep_b_pairs(const CaseConst& C, const double* 
         if (SAMEPV) {
             // kp (P_i + P_j) w / r + kv mu ud w / r^3 = (w / r) (kp (P_i + P_j) + kv mu ud / r^2),
             // kv mu_ij from the table, and w / r = (1 - r/h) / r = max(1/r - 1/h, 0)
-            const double kvmu = C.mu_uniform ? kvmu_i : tab[32 + ti * 4 + tj];
+            // kv mu_ij of the type pair (with one viscosity for every type all entries are kv mu_i)
+            const double kvmu = tab[32 + ti * 4 + tj];
             c = fmax(ir - C.inv_hp, 0.0) * fma(kvmu, ud * (ir * ir), fma(kp, q.v.w, kp_pi));
         } else {
             const double wp = fmax(fma(-r, C.inv_hp, 1.0), 0.0);

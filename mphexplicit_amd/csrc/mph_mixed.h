@@ -245,7 +245,7 @@ __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float*
     const float dcg = (float)C.dcg, cg = (float)C.cg;
     const float mu_i = tabf[ti * 4 + ti];
     // SAMEPV: one FP32 sum of the pressure and viscosity terms with their constant factors
-    const float kpf = (float)(C.dcp * C.vol), kp_pi = kpf * vi.w, kvmu_i = tabf[32 + ti * 4 + ti];
+    const float kpf = (float)(C.dcp * C.vol), kp_pi = kpf * vi.w;
     auto fetch = [&](int j, RecF& r) {
         const double4 p = pos[j];
         r.p = v4f{offset_f<WRAP>(p.x, pf.x, C.hwid[0], C.dwid[0], axes & 1),
@@ -273,7 +273,7 @@ __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float*
         if (SAMEPV) {
             // (w / r) (kp (P_i + P_j) + kv mu ud / r^2) with w / r = max(1/r - 1/h, 0), as the
             // FP64 sweep
-            const float kvmu = C.mu_uniform ? kvmu_i : tabf[32 + ti * 4 + tj];
+            const float kvmu = tabf[32 + ti * 4 + tj];     // kv mu_ij of the type pair
             const float c = fmaxf(ir - inv_hp, 0.f) * fmaf(kvmu, ud * (ir * ir), fmaf(kpf, q.v.w, kp_pi));
             acc.px = fmaf(c, dx, acc.px);
             acc.py = fmaf(c, dy, acc.py);
