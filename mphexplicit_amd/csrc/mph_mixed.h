@@ -45,35 +45,40 @@ constexpr float kFarF = 1.0e18f;
 __device__ __forceinline__ float far_if_self_f(float q2) { return q2 > 0.f ? q2 : 1.0e36f; }
 
 
-template <int K, int NREC, class Fetch, class Eval>
-__device__ __forceinline__ void walk_union_lds_f(const Frame<K>& f, v4f* __restrict__ lds, Fetch fetch, Eval eval)
+// The FP32 counterpart of walk_union_lds (branch-free entries, the far record in padding
+// slots), with NREC planes of 64 float4 records.  Each lane loads its union entry for the
+// next block (load, into a Raw) while the current block is evaluated, and conv turns it into
+// the FP32 record when the lane stores it to LDS.  With Raw = Rec (FP64 data) the loads stay
+// in flight during the evaluation; with Raw = RecF load converts at once (fewer registers)
+// and conv passes the record through.
+template <int K, int NREC, class Raw, class Load, class Conv, class Eval>
+__device__ __forceinline__ void walk_union_lds_f(const Frame<K>& f, v4f* __restrict__ lds, Load load, Conv conv,
+                                                 Eval eval)
 {
-    // the FP32 counterpart of walk_union_lds (branch-free entries, the far record in
-    // padding slots): NREC planes of 64 float4 records
     constexpr int S = Frame<K>::S;
     const int lane = (int)(threadIdx.x & 63);
     const int len = __builtin_amdgcn_readfirstlane(f.len);
     const int blocks = (len + 63) >> 6;
     auto idx = [&](int blk) { const int e = (blk << 6) + lane; return e < len ? f.list[e] : -1; };
     const RecF far = {v4f{kFarF, kFarF, kFarF, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}, v4f{0.f, 0.f, 0.f, 0.f}};
-    RecF r = far;
+    Raw r = {};
     int jr = idx(0), jn = idx(1);
-    if (jr >= 0) fetch(jr, r);
+    bool have = jr >= 0;
+    if (have) load(jr, r);
     for (int blk = 0; blk < blocks; ++blk) {
         // in-order LDS operations of one wavefront; the barriers only stop the compiler
         // from moving LDS accesses across the block boundary
         __builtin_amdgcn_wave_barrier();
-        lds[0 * 64 + lane] = r.p;
-        lds[1 * 64 + lane] = r.v;
-        if (NREC > 2) lds[2 * 64 + lane] = r.a;
+        const RecF s = have ? conv(r) : far;
+        lds[0 * 64 + lane] = s.p;
+        lds[1 * 64 + lane] = s.v;
+        if (NREC > 2) lds[2 * 64 + lane] = s.a;
         __builtin_amdgcn_wave_barrier();
         if (blk + 1 < blocks) {
             jr = jn;
             jn = idx(blk + 2);
-            if (jr >= 0)
-                fetch(jr, r);
-            else
-                r = far;
+            have = jr >= 0;
+            if (have) load(jr, r);
         }
         const int rest = len - (blk << 6);
         const int nsteps = rest >= 64 ? K : (rest + S - 1) / S;
@@ -104,13 +109,19 @@ __device__ __forceinline__ void sweep_a_pairs_f(const CaseConst& C, const float*
     const float hwx = (float)C.hwid[0], hwy = (float)C.hwid[1], hwz = (float)C.hwid[2];
     const float wx = (float)C.dwid[0], wy = (float)C.dwid[1], wz = (float)C.dwid[2];
     const float inv_ha = (float)C.inv_ha, inv_hp = (float)C.inv_hp, hmax2 = (float)C.hmax2;
-    auto fetch = [&](int j, RecF& r) {
-        const double4 p = pos[j];
-        r.p = v4f{offset_f<WRAP>(p.x, pf.x, C.hwid[0], C.dwid[0], axes & 1),
-                  offset_f<WRAP>(p.y, pf.y, C.hwid[1], C.dwid[1], axes & 2),
-                  offset_f<WRAP>(p.z, pf.z, C.hwid[2], C.dwid[2], axes & 4), __int_as_float(tag_of(p.w) & 3)};
+    auto load = [&](int j, Rec& r) {
+        r.p = ld4(pos + j);
         // P (vel.w) of partners is being written by this kernel: read x, y, z only
-        r.v = v4f{(float)vel[j].x, (float)vel[j].y, (float)vel[j].z, 0.f};
+        const double4 u = vel[j];
+        r.v = v4d{u.x, u.y, u.z, 0.0};
+    };
+    auto conv = [&](const Rec& r) {
+        RecF s;
+        s.p = v4f{offset_f<WRAP>(r.p.x, pf.x, C.hwid[0], C.dwid[0], axes & 1),
+                  offset_f<WRAP>(r.p.y, pf.y, C.hwid[1], C.dwid[1], axes & 2),
+                  offset_f<WRAP>(r.p.z, pf.z, C.hwid[2], C.dwid[2], axes & 4), __int_as_float(tag_of(r.p.w) & 3)};
+        s.v = v4f{(float)r.v.x, (float)r.v.y, (float)r.v.z, 0.f};
+        return s;
     };
     auto eval = [&](const RecF& q) {
         float dx = q.p.x - ox, dy = q.p.y - oy, dz = q.p.z - oz;
@@ -138,7 +149,7 @@ __device__ __forceinline__ void sweep_a_pairs_f(const CaseConst& C, const float*
         const float ud = fmaf(q.v.z - viz, dz, fmaf(q.v.y - viy, dy, (q.v.x - vix) * dx));
         acc.ds += (double)(ud * (omp * ir));
     };
-    walk_union_lds_f<K, 2>(f, lds, fetch, eval);
+    walk_union_lds_f<K, 2, Rec>(f, lds, load, conv, eval);
 }
 
 // Without surface tension the mixed sweeps are held to seven wavefronts per SIMD (72
@@ -246,7 +257,8 @@ __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float*
     const float mu_i = tabf[ti * 4 + ti];
     // SAMEPV: one FP32 sum of the pressure and viscosity terms with their constant factors
     const float kpf = (float)(C.dcp * C.vol), kp_pi = kpf * vi.w;
-    auto fetch = [&](int j, RecF& r) {
+    // converted as loaded: measured faster here than keeping the FP64 loads in flight
+    auto load = [&](int j, RecF& r) {
         const double4 p = pos[j];
         r.p = v4f{offset_f<WRAP>(p.x, pf.x, C.hwid[0], C.dwid[0], axes & 1),
                   offset_f<WRAP>(p.y, pf.y, C.hwid[1], C.dwid[1], axes & 2),
@@ -258,6 +270,7 @@ __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float*
             r.a = v4f{(float)a.x, (float)a.y, (float)a.z, (float)a.w};
         }
     };
+    auto conv = [](const RecF& r) { return r; };
     auto eval = [&](const RecF& q) {
         float dx = q.p.x - ox, dy = q.p.y - oy, dz = q.p.z - oz;
         if (WRAP == 2) {
@@ -309,7 +322,7 @@ __device__ __forceinline__ void sweep_b_pairs_f(const CaseConst& C, const float*
             acc.sz = fmaf(-om, fmaf(qd, dz, wg * qz), acc.sz);
         }
     };
-    walk_union_lds_f<K, SURF ? 3 : 2>(f, lds, fetch, eval);
+    walk_union_lds_f<K, SURF ? 3 : 2, RecF>(f, lds, load, conv, eval);
 }
 
 template <int K>
