@@ -336,6 +336,22 @@ def wide_dam_break_3d(seed=15) -> Deck:
                         name="wide_dambreak3d")
 
 
+def periodic_flow_3d(n_side=20, l0=1.0e-3, drift=(0.02, 0.05, 0.0), seed=16, steps=40, name=None) -> Deck:
+    """Fluid filling a periodic cube (no walls, no gravity) that drifts as a whole with a
+    random perturbation: particles, and the members of a cluster, cross the periodic faces
+    between rebuilds (every axis wide enough for the sweeps' image path)."""
+    rng = np.random.default_rng(seed)
+    rho, mu = 1000.0, 1.0e-3
+    vmax = float(np.linalg.norm(drift))
+    K, dt = sound_speed_params(rho, 10.0 * vmax, l0)
+    x = _jitter(rng, _lattice((n_side,) * 3, l0, (0.0, 0.0, 0.0)), 0.01 * l0, 3)
+    v = np.asarray(drift, np.float64)[None, :] + rng.uniform(-0.2, 0.2, size=x.shape) * vmax
+    data = DataParams(dt=dt, end_time=dt * steps, density=(rho,) * 4, bulk_modulus=(K,) * 4,
+                      bulk_viscosity=(mu,) * 4, shear_viscosity=(mu,) * 4, surface_tension=(0.0,) * 4)
+    return Deck(name or "periodic_flow3d", 3, data, l0, np.zeros(3), np.full(3, n_side * l0),
+                np.zeros(len(x), np.int32), x, v, steps=steps)
+
+
 def drift_dam_break_3d(seed=14) -> Deck:
     """Small 3D dam break for 1000-step runs: 8x11x8 fluid in a 14x11x12 walled box."""
     return dam_break_3d(fluid=(8, 11, 8), box=(14, 11, 12), jitter=0.01, seed=seed, steps=1000,

@@ -24,6 +24,9 @@ DECKS = {
     "twofluid3d": decks.small_two_fluid_3d,
     # every width above eight cut-offs: the list build's bounding-box test is on in 3D
     "wide3d": decks.wide_dam_break_3d,
+    # fully periodic drifting fluid: cluster members cross the faces between rebuilds while
+    # the sweeps take images near each cluster's first member
+    "periodicflow3d": decks.periodic_flow_3d,
 }
 
 COMPARED = ("x", "v", "force", "dens_a", "vol_strain", "divergence", "press_p", "press_a", "gcenter")
@@ -107,7 +110,8 @@ def test_first_step_parity(solver_cls, name):
     assert s.time() == orc.time
 
 
-@pytest.mark.parametrize("name", ["droplet2d", "movingwall2d", "bridge2d", "squaredrop2d", "twofluid3d", "wide3d"])
+@pytest.mark.parametrize("name", ["droplet2d", "movingwall2d", "bridge2d", "squaredrop2d", "twofluid3d", "wide3d",
+                                  "periodicflow3d"])
 def test_multi_step_parity(solver_cls, name):
     print("case", name, flush=True)
     d = DECKS[name]()
@@ -190,7 +194,7 @@ def test_golden_fixtures(solver_cls, name):
 
 
 @pytest.mark.parametrize("K", ["2", "4", "8"])
-@pytest.mark.parametrize("name", ["twofluid3d", "movingwall2d", "wide3d"])
+@pytest.mark.parametrize("name", ["twofluid3d", "movingwall2d", "wide3d", "periodicflow3d"])
 def test_every_cluster_size(solver_cls, name, K, monkeypatch):
     # each tile shape the autotuner can pick gives the reference's lists and fields
     monkeypatch.setenv("MPH_TUNE", K)
