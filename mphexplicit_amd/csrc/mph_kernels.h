@@ -121,14 +121,6 @@ __device__ __forceinline__ double rsqrt_pair(double q2)
 constexpr double kFar = 1.0e30;
 __device__ __forceinline__ double far_if_self(double q2) { return q2 > 0.0 ? q2 : 1.0e300; }
 
-// true if no neighbour-list partner of a particle at p can need a periodic image on any
-// axis: p is at least `guard` (list cut-off + Verlet skin) inside every face
-__device__ __forceinline__ bool away_from_faces(const CaseConst& C, const double4& p)
-{
-    return p.x - C.dmin[0] >= C.guard[0] && C.dmax[0] - p.x >= C.guard[0] &&
-           p.y - C.dmin[1] >= C.guard[1] && C.dmax[1] - p.y >= C.guard[1] &&
-           p.z - C.dmin[2] >= C.guard[2] && C.dmax[2] - p.z >= C.guard[2];
-}
 
 // -------------------------------------------------------------------------------------
 // move: convection (fluid), rigid wall motion, periodic wrap, Verlet trigger
@@ -916,15 +908,11 @@ __device__ __forceinline__ int slot_sum_i(int v)
     return v;
 }
 
-// true if no list partner of a particle at p can need a periodic image on any axis
-__device__ __forceinline__ bool away(const CaseConst& C, const double4& p)
-{
-    return away_from_faces(C, p);
-}
-
 // Periodic path of a wavefront: 0 if no row is near a face (plain differences), 1 if the
 // axes its rows are near all take images near the first member, 2 otherwise (the exact
-// periodic difference per row and pair).  axes: the axes that need images (uniform).
+// periodic difference per row and pair).  axes: the axes that need images (uniform).  A
+// particle at least `guard` (list cut-off + Verlet skin) inside both faces of an axis has
+// no list partner that wraps on that axis.
 __device__ __forceinline__ int wrap_path(const CaseConst& C, const double4& p, int& axes)
 {
     const bool nx = !(p.x - C.dmin[0] >= C.guard[0] && C.dmax[0] - p.x >= C.guard[0]);
