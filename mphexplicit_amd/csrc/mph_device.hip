@@ -95,6 +95,7 @@ struct mph_ctx {
     DevBuf<int2> d_bmap, d_cmap;
     DevBuf<double4> pos, pos_alt, vel, vel_alt, vnext, aux, force, snap;
     DevBuf<int> pidx, pidx_alt, nout, start;
+    DevBuf<int> selfin;            // 1: the particle's own index is in its cluster's union list
     DevBuf<double> dens_a, vstrain, divp, wallc, vir;
     DevBuf<unsigned> keys_a, keys_b, hist, ccount, scan_sums;
     DevBuf<int> perm_a, perm_b, rank, cell_of_rank, cell_case, slot_case;
@@ -472,7 +473,7 @@ static int launch_rebuild(mph_ctx* c, int count_only)
     const int lgrid = std::max(1, std::min((c->nslots + 3) / 4, 4096));
     LAUNCH_K(k_cluster_lists, c->K, dim3((unsigned)lgrid), dim3(256), 0, s, c->d_cases.p,
              c->slot_case.p, c->nslots, c->flags.p, c->any_flag.p, c->pos.p, c->start.p, c->rank.p,
-             c->cl_info.p, c->cl_count.p, count_only ? 0 : c->width, c->cl_nbr.p, c->cl_len.p, c->nout.p,
+             c->cl_info.p, c->cl_count.p, count_only ? 0 : c->width, c->cl_nbr.p, c->cl_len.p, c->selfin.p, c->nout.p,
              c->max_len.p);
     return launch_ok("rebuild");
 }
@@ -492,7 +493,7 @@ static int launch_sweep_a(mph_ctx* c, int real, bool count = false)
 #define SWEEP_A_ARGS                                                                                        \
     dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p,  \
         c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p,          \
-        c->vstrain.p, c->divp.p, real ? c->pairs.p : c->pairs_scratch.p
+        c->vstrain.p, c->divp.p, real ? c->pairs.p : c->pairs_scratch.p, c->selfin.p
     const bool cnt = count && real;
 #define SWEEP_A_MODE(M)                                                                            \
     do {                                                                                           \
@@ -826,7 +827,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         (rc = c->pos.alloc(n)) || (rc = c->pos_alt.alloc(n)) || (rc = c->vel.alloc(n)) ||
         (rc = c->vel_alt.alloc(n)) || (rc = c->vnext.alloc(n)) || (rc = c->aux.alloc(n)) ||
         (rc = c->force.alloc(n)) || (rc = c->snap.alloc(n)) || (rc = c->pidx.alloc(n)) ||
-        (rc = c->pidx_alt.alloc(n)) || (rc = c->nout.alloc(n)) ||
+        (rc = c->pidx_alt.alloc(n)) || (rc = c->nout.alloc(n)) || (rc = c->selfin.alloc(n)) ||
         (rc = c->start.alloc((size_t)c->nkeys + 1)) || (rc = c->dens_a.alloc(n)) ||
         (rc = c->vstrain.alloc(n)) || (rc = c->divp.alloc(n)) || (rc = c->vir.alloc(n)) ||
         (rc = c->wallc.alloc(hwall.size())) || (rc = c->keys_a.alloc(n)) || (rc = c->keys_b.alloc(n)) ||

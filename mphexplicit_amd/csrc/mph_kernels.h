@@ -562,9 +562,10 @@ template <int K>
 __device__ __forceinline__ void cluster_list_one(const CaseConst& C, int g, const int4 info,
                                                  const double4* __restrict__ pos, const int* __restrict__ start,
                                                  const int* __restrict__ rank, int width, int* __restrict__ cl_nbr,
-                                                 int* __restrict__ cl_len, int* __restrict__ count_out,
-                                                 int* __restrict__ max_len, double4* cand, int* candj, int* candf,
-                                                 int* srank, int* clo, int* cmid, int* cpre)
+                                                 int* __restrict__ cl_len, int* __restrict__ selfin,
+                                                 int* __restrict__ count_out, int* __restrict__ max_len,
+                                                 double4* cand, int* candj, int* candf, int* srank, int* clo,
+                                                 int* cmid, int* cpre)
 {
     constexpr int S = 64 / K;
     const int lane = threadIdx.x & 63;
@@ -645,6 +646,7 @@ __device__ __forceinline__ void cluster_list_one(const CaseConst& C, int g, cons
     };
     const unsigned long long below = (1ull << lane) - 1ull;
     int count = 0, nfluid = 0, self = 0, ulen = 0;
+    unsigned present = 0;           // bit m: member m is in the union list
     int* out = cl_nbr + (size_t)g * width;
     int fn = 0;
     int jn = lane < total ? locate(lane, fn) : -1;
@@ -689,6 +691,8 @@ __device__ __forceinline__ void cluster_list_one(const CaseConst& C, int g, cons
             if (row == 0 && ((u >> slot) & 1ull)) {
                 const int at = ulen + (int)__popcll(u & ((1ull << slot) - 1ull));
                 if (at < width) out[at] = j;
+                const unsigned m = (unsigned)(j - info.x);
+                if (m < (unsigned)info.y) present |= 1u << m;
             }
             ulen += (int)__popcll(u);
         }
@@ -698,7 +702,9 @@ __device__ __forceinline__ void cluster_list_one(const CaseConst& C, int g, cons
         nfluid += __shfl_xor(nfluid, off, S);
         self |= __shfl_xor(self, off, S);
     }
+    for (int off = 32; off > 0; off >>= 1) present |= __shfl_xor(present, off);
     if (mine && slot == 0) count_out[i] = (wall_i && nfluid == 0) ? 0 : count + self;
+    if (lane < info.y) selfin[info.x + lane] = (present >> lane) & 1u;
     if (lane == 0) {
         cl_len[g] = ulen < width ? ulen : width;
         atomicMax(max_len, ulen);
@@ -713,7 +719,8 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
                 const double4* __restrict__ pos, const int* __restrict__ start,
                 const int* __restrict__ rank, const int4* __restrict__ cl_info,
                 const int* __restrict__ cl_count, int width, int* __restrict__ cl_nbr,
-                int* __restrict__ cl_len, int* __restrict__ count_out, int* __restrict__ max_len)
+                int* __restrict__ cl_len, int* __restrict__ selfin, int* __restrict__ count_out,
+                int* __restrict__ max_len)
 {
     if (*any_flag == 0) return;
     __shared__ double4 cand[4][64];
@@ -724,7 +731,7 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
     for (int g = blockIdx.x * 4 + wv; g < nslots; g += gridDim.x * 4) {
         const int c = slot_case[g];
         if (flags[c] == 0 || g - cases[c].cl_base >= cl_count[c]) continue;     // wave-uniform
-        cluster_list_one<K>(cases[c], g, cl_info[g], pos, start, rank, width, cl_nbr, cl_len, count_out, max_len,
+        cluster_list_one<K>(cases[c], g, cl_info[g], pos, start, rank, width, cl_nbr, cl_len, selfin, count_out, max_len,
                             cand[wv], candj[wv], candf[wv], srank[wv], clo[wv], cmid[wv], cpre[wv]);
     }
 }
@@ -989,7 +996,11 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         const double dx = WRAP == 2 ? rel_wrap(q.p.x, pi.x, C.hwid[0], C.dwid[0]) : q.p.x - pi.x;
         const double dy = WRAP == 2 ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : q.p.y - pi.y;
         const double dz = WRAP == 2 ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : q.p.z - pi.z;
-        const double q2 = far_if_self(fma(dz, dz, fma(dy, dy, dx * dx)));
+        // a row's own entry (x_ij = 0 exactly) is kept off q^2 = 0: it then adds 1 to the
+        // sum of w_P and to the pair count, which the epilogue takes back (selfin), a weight
+        // sa = r/h of about 1e-147 to n^A, below every other term's last bit, and exact
+        // zeros elsewhere (x_ij = 0, u.x_ij = 0)
+        const double q2 = fmax(fma(dz, dz, fma(dy, dy, dx * dx)), 1.0e-300);
         if (COUNT) acc.np += q2 <= C.hmax2;
         const double ir = rsqrt_pair(q2);
         const double r = q2 * ir;
@@ -1027,7 +1038,7 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int* __restrict__ count_out,
           const double4* __restrict__ pos, double4* vel, double4* __restrict__ aux,
           double* __restrict__ dens_a, double* __restrict__ vstrain, double* __restrict__ divp,
-          unsigned long long* __restrict__ pair_count)
+          unsigned long long* __restrict__ pair_count, const int* __restrict__ selfin)
 {
     // no side effect beyond its outputs: the eager step may run it once more after a rebuild
     const int2 m = cmap[blockIdx.x];
@@ -1073,6 +1084,9 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     int npair = COUNT ? slot_sum_i<K>(acc.np) : 0;
     if (f.slot == 0 && f.valid) {
         const int i = f.i;
+        const int own = selfin[i];             // the row's own entry: w_P(0) = 1 and one pair
+        acc.ws -= (double)own;
+        npair -= own;
         if (count_out[i] == 0)                 // no list (main.cpp:1416): all sums are zero
             acc = AccA{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
         const double na = acc.na * C.ca;
