@@ -486,7 +486,7 @@ static int launch_sweep_a(mph_ctx* c, int real, bool count = false)
         LAUNCH_SK(k_sweep_a_mixed, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream,
                   c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width,
                   c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p,
-                  real ? c->pairs.p : c->pairs_scratch.p);
+                  real ? c->pairs.p : c->pairs_scratch.p, c->selfin.p);
         return launch_ok("sweep A");
     }
     const int mode = (c->same_ap ? 1 : 0) | (c->ratio_one ? 2 : 0);

@@ -130,7 +130,9 @@ __device__ __forceinline__ void sweep_a_pairs_f(const CaseConst& C, const float*
             dy = fold_f(dy, hwy, wy);
             dz = fold_f(dz, hwz, wz);
         }
-        const float q2 = far_if_self_f(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+        // a row's own entry (zero offset difference) is kept off q^2 = 0 as in k_sweep_a; its
+        // w_P(0) = 1 and pair count are taken back in the epilogue (selfin)
+        const float q2 = fmaxf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)), 1.0e-36f);
         acc.np += q2 <= hmax2;
         const float ir = __builtin_amdgcn_rsqf(q2);
         const float r = q2 * ir;
@@ -163,7 +165,7 @@ k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
                 const int* __restrict__ count_out,
                 const double4* __restrict__ pos, double4* vel, double4* __restrict__ aux,
                 double* __restrict__ dens_a, double* __restrict__ vstrain, double* __restrict__ divp,
-                unsigned long long* __restrict__ pair_count)
+                unsigned long long* __restrict__ pair_count, const int* __restrict__ selfin)
 {
     const int2 m = cmap[blockIdx.x];
     if (m.x < 0) return;
@@ -209,6 +211,9 @@ k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
     int npair = slot_sum_i<K>(acc.np);
     if (f.slot == 0 && f.valid) {
         const int i = f.i;
+        const int own = selfin[i];
+        acc.ws -= (double)own;
+        npair -= own;
         if (count_out[i] == 0)                 // no list (main.cpp:1416): all sums are zero
             acc = AccA{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
         const double na = acc.na * C.ca;
