@@ -115,9 +115,10 @@ __device__ __forceinline__ double rsqrt_pair(double q2)
 
 // The pair sweeps evaluate every union entry without a branch.  Padding entries carry a
 // record far outside every support (kFar), and a row's own entry has q^2 == 0 exactly (the
-// same double on both sides); far_if_self maps that q^2 to one outside every support, so
-// both contribute exact zeros to every sum (0 * finite).  Two distinct particles at the
-// same position would make the reference divide by zero; here they do not interact.
+// same double on both sides).  far_if_self maps that q^2 to one outside every support, so
+// both contribute exact zeros to every sum (0 * finite); sweeps A and B instead keep q^2 off
+// zero with one max and handle the own entry's terms themselves.  Two distinct particles at
+// the same position would make the reference divide by zero; the sweeps stay finite.
 constexpr double kFar = 1.0e30;
 __device__ __forceinline__ double far_if_self(double q2) { return q2 > 0.0 ? q2 : 1.0e300; }
 
