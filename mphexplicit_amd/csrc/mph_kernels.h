@@ -998,18 +998,17 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         const double dy = WRAP == 2 ? rel_wrap(q.p.y, pi.y, C.hwid[1], C.dwid[1]) : q.p.y - pi.y;
         const double dz = WRAP == 2 ? rel_wrap(q.p.z, pi.z, C.hwid[2], C.dwid[2]) : q.p.z - pi.z;
         // a row's own entry (x_ij = 0 exactly) is kept off q^2 = 0: it then adds 1 to the
-        // sum of w_P and to the pair count, which the epilogue takes back (selfin), a weight
-        // sa = r/h of about 1e-147 to n^A, below every other term's last bit, and exact
-        // zeros elsewhere (x_ij = 0, u.x_ij = 0)
+        // sum of w_P and to the pair count, which the epilogue takes back (selfin), r of
+        // about 1e-150 to the n^A sum, below every other term's last bit, and exact zeros
+        // elsewhere (x_ij = 0, u.x_ij = 0)
         const double q2 = fmax(fma(dz, dz, fma(dy, dy, dx * dx)), 1.0e-300);
         if (COUNT) acc.np += q2 <= C.hmax2;
         const double ir = rsqrt_pair(q2);
         const double r = q2 * ir;
-        const double sa = r * C.inv_ha;
-        const double oma = fmax(1.0 - sa, 0.0);
+        const double oma = fmax(fma(-r, C.inv_ha, 1.0), 0.0);
         const double ratio = (R1 || C.ratio_one) ? 1.0 : tab[16 + ti * 4 + (tag_of(q.p.w) & 3)];
         const double wa = ratio * (oma * oma);
-        acc.na = fma(wa, sa, acc.na);
+        acc.na = fma(wa, r, acc.na);          // sum of w_A r; 1/h_A joins the prefactor
         if (SURF) {
             acc.gx = fma(dx, wa, acc.gx);
             acc.gy = fma(dy, wa, acc.gy);
@@ -1090,7 +1089,7 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
         npair -= own;
         if (count_out[i] == 0)                 // no list (main.cpp:1416): all sums are zero
             acc = AccA{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
-        const double na = acc.na * C.ca;
+        const double na = acc.na * C.inv_ha * C.ca;
         const double ep = C.cp * acc.ws - C.n0p;
         const double dv = -C.dcp * acc.ds;
         const double kap = ep < 0.0 ? 0.0 : C.kbulk[ti];
