@@ -352,6 +352,23 @@ def periodic_flow_3d(n_side=20, l0=1.0e-3, drift=(0.02, 0.05, 0.0), seed=16, ste
                 np.zeros(len(x), np.int32), x, v, steps=steps)
 
 
+def sparse_random_3d(n=1500, side=24, l0=1.0e-3, seed=17, steps=10, name=None) -> Deck:
+    """Particles at uniformly random positions and velocities in a periodic cube, about 0.11
+    per l0^3 (a tenth of the lattice density), no gravity: cells hold a few particles far apart,
+    so some members of a cluster have no other member within the cut-off, and some pairs are
+    much closer than l0."""
+    rng = np.random.default_rng(seed)
+    rho, mu = 1000.0, 1.0e-3
+    K, dt = sound_speed_params(rho, 1.0, l0)
+    L = side * l0
+    x = rng.uniform(0.0, L, size=(n, 3))
+    data = DataParams(dt=dt, end_time=dt * steps, density=(rho,) * 4, bulk_modulus=(K,) * 4,
+                      bulk_viscosity=(mu,) * 4, shear_viscosity=(mu,) * 4, surface_tension=(0.0,) * 4)
+    v = rng.uniform(-0.05, 0.05, size=(n, 3))
+    return Deck(name or "sparse_random3d", 3, data, l0, np.zeros(3), np.full(3, L), np.zeros(n, np.int32), x,
+                v, steps=steps)
+
+
 def drift_dam_break_3d(seed=14) -> Deck:
     """Small 3D dam break for 1000-step runs: 8x11x8 fluid in a 14x11x12 walled box."""
     return dam_break_3d(fluid=(8, 11, 8), box=(14, 11, 12), jitter=0.01, seed=seed, steps=1000,

@@ -27,6 +27,9 @@ DECKS = {
     # fully periodic drifting fluid: cluster members cross the faces between rebuilds while
     # the sweeps take images near each cluster's first member
     "periodicflow3d": decks.periodic_flow_3d,
+    # a tenth of the lattice density at random positions: cluster members with no other
+    # member within the cut-off (their own index is not in the union list)
+    "sparse3d": decks.sparse_random_3d,
 }
 
 COMPARED = ("x", "v", "force", "dens_a", "vol_strain", "divergence", "press_p", "press_a", "gcenter")
@@ -194,7 +197,7 @@ def test_golden_fixtures(solver_cls, name):
 
 
 @pytest.mark.parametrize("K", ["2", "4", "8"])
-@pytest.mark.parametrize("name", ["twofluid3d", "movingwall2d", "wide3d", "periodicflow3d"])
+@pytest.mark.parametrize("name", ["twofluid3d", "movingwall2d", "wide3d", "periodicflow3d", "sparse3d"])
 def test_every_cluster_size(solver_cls, name, K, monkeypatch):
     # each tile shape the autotuner can pick gives the reference's lists and fields
     monkeypatch.setenv("MPH_TUNE", K)
