@@ -686,8 +686,9 @@ __device__ __forceinline__ void cluster_list_one(const CaseConst& C, int g, cons
             count += other;
             nfluid += other && candf[e];
             const unsigned long long bv = __ballot(other);
-            unsigned long long u = 0;
-            for (int r = 0; r < K; ++r) u |= bv >> (r * S);
+            // OR of the K row chunks of S bits, folded in log2(K) halvings
+            unsigned long long u = bv;
+            for (int sh = 32; sh >= S; sh >>= 1) u |= u >> sh;
             u &= (S == 64 ? ~0ull : ((1ull << S) - 1ull));
             if (row == 0 && ((u >> slot) & 1ull)) {
                 const int at = ulen + (int)__popcll(u & ((1ull << slot) - 1ull));
