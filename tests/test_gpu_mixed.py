@@ -23,6 +23,10 @@ DECKS = {
     "movingwall2d": decks.small_moving_wall_2d,
     "dambreak3d": decks.small_dam_break_3d,
     "twofluid3d": decks.small_two_fluid_3d,
+    # periodic axes wide enough for images near the cluster's first member (no fold per
+    # pair), with cluster members crossing the faces between rebuilds in the drifting deck
+    "wide3d": decks.wide_dam_break_3d,
+    "periodicflow3d": decks.periodic_flow_3d,
 }
 
 # per-field bounds after one step and after 20 steps (normwise, see above)
@@ -71,7 +75,7 @@ def test_mixed_first_step(solver_mod, name):
     assert not bad, bad
 
 
-@pytest.mark.parametrize("name", ["droplet2d", "twofluid3d"])
+@pytest.mark.parametrize("name", ["droplet2d", "twofluid3d", "periodicflow3d"])
 def test_mixed_multi_step(solver_mod, name):
     d = DECKS[name]()
     s64 = solver_mod.Solver.from_deck(d, precision=solver_mod.MPH_FP64)
