@@ -326,6 +326,12 @@ static int derive_case(const mph_params& p, int begin, int end, int nfluid, int 
     k.img_axes = 0;
     for (int d = 0; d < 3; ++d)
         if (k.hwid[d] > k.cw[d] + reach) k.img_axes |= 1 << d;
+    // plain differences round within 2^-52 (W/2 + cut-off) of the reference's periodic form,
+    // under 1e-9 of the cut-off for W/2 below 1e6 cut-offs: the list build's plain test
+    // redoes the reference's arithmetic for q^2 within 1e-9 of the cut-off
+    k.list_plain = 1;
+    for (int d = 0; d < 3; ++d)
+        if (!(k.hwid[d] <= 1.0e6 * (maxr + margin))) k.list_plain = 0;
     k.hmax2 = maxr * maxr;
     k.shift2 = 0.5 * margin * 0.5 * margin;                             // main.cpp:1102
     return MPH_OK;

@@ -560,7 +560,7 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
 // few clusters rebuild.
 
 template <int K>
-__device__ __forceinline__ void cluster_list_one(const CaseConst& C, int g, const int4 info,
+__device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const int4 info,
                                                  const double4* __restrict__ pos, const int* __restrict__ start,
                                                  const int* __restrict__ rank, int width, int* __restrict__ cl_nbr,
                                                  int* __restrict__ cl_len, int* __restrict__ selfin,
@@ -575,11 +575,28 @@ __device__ __forceinline__ void cluster_list_one(const CaseConst& C, int g, cons
     const int i = info.x + (mine ? row : 0);
     const double4 pi = pos[i];
     const bool wall_i = (tag_of(pi.w) & 3) >= 2;
+    // A cluster whose members are all at least `guard` inside both faces of every axis has
+    // no list pair that wraps (the partners across a face are farther than the guard, which
+    // exceeds the cut-off, in the plain difference too).  Such a wavefront (plain, uniform)
+    // decides pairs by the plain difference, and redoes the reference's periodic arithmetic
+    // only for q^2 within 1e-9 of the cut-off, where the two can round apart.
+    const bool nearf = !(pi.x - C.dmin[0] >= C.guard[0] && C.dmax[0] - pi.x >= C.guard[0] &&
+                         pi.y - C.dmin[1] >= C.guard[1] && C.dmax[1] - pi.y >= C.guard[1] &&
+                         pi.z - C.dmin[2] >= C.guard[2] && C.dmax[2] - pi.z >= C.guard[2]);
+    const bool plain = C.list_plain && __ballot(nearf) == 0ull;
     // members' bounding box, relative to the first member
     const double4 pf = pos[info.x];
-    double bx0 = rel_exact(pi.x, pf.x, C.hwid[0], C.dwid[0]), bx1 = bx0;
-    double by0 = rel_exact(pi.y, pf.y, C.hwid[1], C.dwid[1]), by1 = by0;
-    double bz0 = rel_exact(pi.z, pf.z, C.hwid[2], C.dwid[2]), bz1 = bz0;
+    double bx0, by0, bz0;
+    if (plain) {
+        bx0 = pi.x - pf.x;
+        by0 = pi.y - pf.y;
+        bz0 = pi.z - pf.z;
+    } else {
+        bx0 = rel_exact(pi.x, pf.x, C.hwid[0], C.dwid[0]);
+        by0 = rel_exact(pi.y, pf.y, C.hwid[1], C.dwid[1]);
+        bz0 = rel_exact(pi.z, pf.z, C.hwid[2], C.dwid[2]);
+    }
+    double bx1 = bx0, by1 = by0, bz1 = bz0;
     for (int off = 32; off > 0; off >>= 1) {
         bx0 = fmin(bx0, __shfl_xor(bx0, off)); bx1 = fmax(bx1, __shfl_xor(bx1, off));
         by0 = fmin(by0, __shfl_xor(by0, off)); by1 = fmax(by1, __shfl_xor(by1, off));
@@ -594,9 +611,16 @@ __device__ __forceinline__ void cluster_list_one(const CaseConst& C, int g, cons
                          (C.dim == 2 || C.dwid[2] * C.dwid[2] >= w8);
     auto near_box = [&](const double4& p) {
         if (!use_box) return true;
-        const double dx = rel_exact(p.x, pf.x, C.hwid[0], C.dwid[0]);
-        const double dy = rel_exact(p.y, pf.y, C.hwid[1], C.dwid[1]);
-        const double dz = rel_exact(p.z, pf.z, C.hwid[2], C.dwid[2]);
+        double dx, dy, dz;
+        if (plain) {
+            dx = p.x - pf.x;
+            dy = p.y - pf.y;
+            dz = p.z - pf.z;
+        } else {
+            dx = rel_exact(p.x, pf.x, C.hwid[0], C.dwid[0]);
+            dy = rel_exact(p.y, pf.y, C.hwid[1], C.dwid[1]);
+            dz = rel_exact(p.z, pf.z, C.hwid[2], C.dwid[2]);
+        }
         const double ex = fmax(fmax(dx - bx1, bx0 - dx), 0.0);
         const double ey = fmax(fmax(dy - by1, by0 - dy), 0.0);
         const double ez = fmax(fmax(dz - bz1, bz0 - dz), 0.0);
@@ -676,10 +700,23 @@ __device__ __forceinline__ void cluster_list_one(const CaseConst& C, int g, cons
             const int e = t * S + slot;
             const int j = candj[e];
             const double4 pj = cand[e];
-            const double dx = rel_exact(pj.x, pi.x, C.hwid[0], C.dwid[0]);
-            const double dy = rel_exact(pj.y, pi.y, C.hwid[1], C.dwid[1]);
-            const double dz = rel_exact(pj.z, pi.z, C.hwid[2], C.dwid[2]);
-            const double q2 = dx * dx + dy * dy + dz * dz;
+            double q2;
+            if (plain) {
+                const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                q2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                const bool edge = fabs(q2 - C.cut2) <= 1.0e-9 * C.cut2;
+                if (__ballot(edge) != 0ull && edge) {
+                    const double ex = rel_exact(pj.x, pi.x, C.hwid[0], C.dwid[0]);
+                    const double ey = rel_exact(pj.y, pi.y, C.hwid[1], C.dwid[1]);
+                    const double ez = rel_exact(pj.z, pi.z, C.hwid[2], C.dwid[2]);
+                    q2 = ex * ex + ey * ey + ez * ez;
+                }
+            } else {
+                const double dx = rel_exact(pj.x, pi.x, C.hwid[0], C.dwid[0]);
+                const double dy = rel_exact(pj.y, pi.y, C.hwid[1], C.dwid[1]);
+                const double dz = rel_exact(pj.z, pi.z, C.hwid[2], C.dwid[2]);
+                q2 = dx * dx + dy * dy + dz * dz;
+            }
             const bool in = mine && e < ns && q2 <= C.cut2;
             const bool other = in && j != i;
             self |= in && j == i;
@@ -707,11 +744,9 @@ __device__ __forceinline__ void cluster_list_one(const CaseConst& C, int g, cons
     for (int off = 32; off > 0; off >>= 1) present |= __shfl_xor(present, off);
     if (mine && slot == 0) count_out[i] = (wall_i && nfluid == 0) ? 0 : count + self;
     if (lane < info.y) selfin[info.x + lane] = (present >> lane) & 1u;
-    if (lane == 0) {
-        cl_len[g] = ulen < width ? ulen : width;
-        atomicMax(max_len, ulen);
-    }
+    if (lane == 0) cl_len[g] = ulen < width ? ulen : width;
     __builtin_amdgcn_wave_barrier();       // this cluster's LDS reads precede the next one's writes
+    return ulen;                           // the caller keeps the largest for the capacity check
 }
 
 template <int K>
@@ -729,13 +764,20 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
     __shared__ int candj[4][64];         // particle index
     __shared__ int candf[4][64];         // 1: in the fluid block of its cell
     __shared__ int srank[4][32], clo[4][32], cmid[4][32], cpre[4][33];
-    const int wv = threadIdx.x >> 6;
+    // the wavefront's slot and case are uniform: readfirstlane lets the compiler keep the
+    // case's constants in scalar registers (scalar loads) instead of vector loads in the loop
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    int longest = 0;
     for (int g = blockIdx.x * 4 + wv; g < nslots; g += gridDim.x * 4) {
-        const int c = slot_case[g];
+        const int c = __builtin_amdgcn_readfirstlane(slot_case[g]);
         if (flags[c] == 0 || g - cases[c].cl_base >= cl_count[c]) continue;     // wave-uniform
-        cluster_list_one<K>(cases[c], g, cl_info[g], pos, start, rank, width, cl_nbr, cl_len, selfin, count_out, max_len,
-                            cand[wv], candj[wv], candf[wv], srank[wv], clo[wv], cmid[wv], cpre[wv]);
+        longest = max(longest, cluster_list_one<K>(cases[c], g, cl_info[g], pos, start, rank, width, cl_nbr, cl_len,
+                                                   selfin, count_out, max_len, cand[wv], candj[wv], candf[wv],
+                                                   srank[wv], clo[wv], cmid[wv], cpre[wv]));
     }
+    // one atomic per wavefront: an atomic per cluster on the one counter serialised the
+    // kernel (about 4 ms for 318k clusters)
+    if ((threadIdx.x & 63) == 0 && longest > 0) atomicMax(max_len, longest);
 }
 
 // -------------------------------------------------------------------------------------
