@@ -556,7 +556,7 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
 // the reference counts it (main.cpp:1408-1420).  Persistent wavefronts: a step without a
 // rebuild costs a small grid, and a rebuild of a few clusters runs them in parallel.  One
 // thread per member walking the candidates with a dependent load each measured 5.5-6 ms per
-// full rebuild of the 1.27M-particle deck (this kernel: 4-4.7 ms) and was latency-bound when
+// full rebuild of the 1.27M-particle deck (this kernel: 2.4-2.5 ms) and was latency-bound when
 // few clusters rebuild.
 
 template <int K>
