@@ -474,9 +474,14 @@ static int launch_rebuild(mph_ctx* c, int count_only)
     hipLaunchKernelGGL(k_cluster_fill, dim3(gcells), dim3(256), 0, s, c->d_cases.p, c->cell_case.p,
                        c->ncells, c->start.p, c->cell_of_rank.p, c->K, c->ccount.p, c->cl_info.p,
                        c->cl_count.p, c->any_flag.p);
-    // persistent 4-wavefront blocks, at most 4096 (four resident rounds at 120 VGPRs): a step
-    // without a rebuild exits in a few microseconds; fewer blocks measured slower
-    const int lgrid = std::max(1, std::min((c->nslots + 3) / 4, 4096));
+    // 4-wavefront blocks looping over the cluster slots.  A single case rebuilds all its
+    // clusters, and more resident rounds balance their uneven cost (318k clusters: 2.45 ms at
+    // 4096 blocks, 2.30 at 10240, 2.27 at 20480 = 16 rounds at 5 wavefronts per SIMD).  In a
+    // batch a rebuild usually covers a few cases, and the blocks that only skip slots cost
+    // more than the balance gains (64-case batch: rebuild phase 0.60 ms at 4096, 0.73 at
+    // 20480).  A step without a rebuild exits in about 5 microseconds either way.
+    const int lcap = c->ncases == 1 ? 20480 : 4096;
+    const int lgrid = std::max(1, std::min((c->nslots + 3) / 4, lcap));
     LAUNCH_K(k_cluster_lists, c->K, dim3((unsigned)lgrid), dim3(256), 0, s, c->d_cases.p,
              c->slot_case.p, c->nslots, c->flags.p, c->any_flag.p, c->pos.p, c->start.p, c->rank.p,
              c->cl_info.p, c->cl_count.p, count_only ? 0 : c->width, c->cl_nbr.p, c->cl_len.p, c->selfin.p, c->nout.p,
