@@ -193,6 +193,9 @@ def main():
     ap.add_argument("--sur-blocks", type=int, default=8)
     ap.add_argument("--sur-k", type=int, default=10)
     ap.add_argument("--sur-tol", type=float, default=0.01)
+    ap.add_argument("--late-from", type=int, default=1000,
+                    help="step from which the developed-flow window is timed (lists rebuilt often)")
+    ap.add_argument("--late-steps", type=int, default=200, help="steps of the developed-flow window; 0: skip")
     ap.add_argument("--error-steps", type=int, default=500,
                     help="steps after which the mixed leg's fields are compared with FP64")
     args = ap.parse_args()
@@ -248,6 +251,21 @@ def main():
                               "frac": m_ach / FP32_PEAK_TFLOPS, "kernel": "k_" + m_dom + "_mixed"},
                  "error_vs_fp64_after_steps": n_err, "error_vs_fp64": errs}
         m.close()
+    late = None
+    if args.late_steps > 0:
+        # the same FP64 run later in the flow, where the faster fluid rebuilds the lists every
+        # few steps: the headline window above sees one or two rebuilds
+        cur = s.stats()["steps"]
+        if cur < args.late_from:
+            s.step(args.late_from - cur)
+        s.sync()
+        rb0 = s.stats()["rebuilds"]
+        t0 = time.perf_counter()
+        s.step(args.late_steps)
+        s.sync()
+        el = time.perf_counter() - t0
+        late = {"from_step": max(cur, args.late_from), "steps": args.late_steps, "value": args.late_steps / el,
+                "unit": "steps/s", "rebuilds": s.stats()["rebuilds"] - rb0}
     batch = None if args.no_batch else batch_leg(args, Solver)
     surrogate = None
     if prec == MPH_FP64 and not args.no_surrogate:
@@ -283,6 +301,7 @@ def main():
         "phase_ms": phases,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "developed_flow": late,
         "mixed": mixed,
         "batch64": batch,
         "surrogate": surrogate,
