@@ -559,8 +559,11 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
 // full rebuild of the 1.27M-particle deck (this kernel: 2.4-2.5 ms) and was latency-bound when
 // few clusters rebuild.
 
-template <int K>
-__device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const int4 info,
+// P clusters of one cell per wavefront (P = 2: rows 0..K-1 are the members of cluster g,
+// rows K..2K-1 those of cluster g1): the candidates of the shared stencil are loaded, located
+// and box-tested once for both, and each cluster's rows unite into its own list.
+template <int K, int P>
+__device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const int4 info, int g1, const int4 info1,
                                                  const double4* __restrict__ pos, const int* __restrict__ start,
                                                  const int* __restrict__ rank, int width, int* __restrict__ cl_nbr,
                                                  int* __restrict__ cl_len, int* __restrict__ selfin,
@@ -568,11 +571,14 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
                                                  double4* cand, int* candj, int* candf, int* srank, int* clo,
                                                  int* cmid, int* cpre)
 {
-    constexpr int S = 64 / K;
+    constexpr int S = 64 / (K * P);
     const int lane = threadIdx.x & 63;
     const int row = lane / S, slot = lane % S;
-    const bool mine = row < info.y;
-    const int i = info.x + (mine ? row : 0);
+    const bool second = P == 2 && row >= K;            // row of cluster g1
+    const int mrow = second ? row - K : row;
+    const int4 inf = second ? info1 : info;
+    const bool mine = mrow < inf.y;
+    const int i = inf.x + (mine ? mrow : 0);
     const double4 pi = pos[i];
     const bool wall_i = (tag_of(pi.w) & 3) >= 2;
     // A cluster whose members are all at least `guard` inside both faces of every axis has
@@ -670,9 +676,10 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
         return j;
     };
     const unsigned long long below = (1ull << lane) - 1ull;
-    int count = 0, nfluid = 0, self = 0, ulen = 0;
-    unsigned present = 0;           // bit m: member m is in the union list
+    int count = 0, nfluid = 0, self = 0, ulen = 0, ulen1 = 0;
+    unsigned present = 0, present1 = 0;     // bit m: member m of the cluster is in its union list
     int* out = cl_nbr + (size_t)g * width;
+    int* out1 = cl_nbr + (size_t)(P == 2 ? g1 : g) * width;
     int fn = 0;
     int jn = lane < total ? locate(lane, fn) : -1;
     double4 pn = jn >= 0 ? pos[jn] : make_double4(0.0, 0.0, 0.0, 0.0);
@@ -723,17 +730,29 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
             count += other;
             nfluid += other && candf[e];
             const unsigned long long bv = __ballot(other);
-            // OR of the K row chunks of S bits, folded in log2(K) halvings
-            unsigned long long u = bv;
-            for (int sh = 32; sh >= S; sh >>= 1) u |= u >> sh;
-            u &= (S == 64 ? ~0ull : ((1ull << S) - 1ull));
+            // OR of each cluster's K row chunks of S bits, folded in log2(K) halvings
+            constexpr unsigned long long smask = S == 64 ? ~0ull : ((1ull << S) - 1ull);
+            unsigned long long u = P == 2 ? (bv & 0xffffffffull) : bv, u1 = P == 2 ? (bv >> 32) : 0ull;
+            for (int sh = 32 / P; sh >= S; sh >>= 1) {
+                u |= u >> sh;
+                if (P == 2) u1 |= u1 >> sh;
+            }
+            u &= smask;
+            u1 &= smask;
             if (row == 0 && ((u >> slot) & 1ull)) {
                 const int at = ulen + (int)__popcll(u & ((1ull << slot) - 1ull));
                 if (at < width) out[at] = j;
                 const unsigned m = (unsigned)(j - info.x);
                 if (m < (unsigned)info.y) present |= 1u << m;
             }
+            if (P == 2 && row == K && ((u1 >> slot) & 1ull)) {
+                const int at = ulen1 + (int)__popcll(u1 & ((1ull << slot) - 1ull));
+                if (at < width) out1[at] = j;
+                const unsigned m = (unsigned)(j - info1.x);
+                if (m < (unsigned)info1.y) present1 |= 1u << m;
+            }
             ulen += (int)__popcll(u);
+            if (P == 2) ulen1 += (int)__popcll(u1);
         }
     }
     for (int off = S / 2; off > 0; off >>= 1) {
@@ -741,16 +760,22 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
         nfluid += __shfl_xor(nfluid, off, S);
         self |= __shfl_xor(self, off, S);
     }
-    for (int off = 32; off > 0; off >>= 1) present |= __shfl_xor(present, off);
+    for (int off = 32; off > 0; off >>= 1) {
+        present |= __shfl_xor(present, off);
+        if (P == 2) present1 |= __shfl_xor(present1, off);
+    }
     if (mine && slot == 0) count_out[i] = (wall_i && nfluid == 0) ? 0 : count + self;
     if (lane < info.y) selfin[info.x + lane] = (present >> lane) & 1u;
+    if (P == 2 && lane < info1.y) selfin[info1.x + lane] = (present1 >> lane) & 1u;
     if (lane == 0) cl_len[g] = ulen < width ? ulen : width;
+    if (P == 2 && lane == 0) cl_len[g1] = ulen1 < width ? ulen1 : width;
     __builtin_amdgcn_wave_barrier();       // this cluster's LDS reads precede the next one's writes
-    return ulen;                           // the caller keeps the largest for the capacity check
+    return ulen > ulen1 ? ulen : ulen1;    // the caller keeps the largest for the capacity check
 }
 
+// five wavefronts per SIMD (at most 96 VGPRs)
 template <int K>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slot_case, int nslots,
                 const int* __restrict__ flags, const int* __restrict__ any_flag,
                 const double4* __restrict__ pos, const int* __restrict__ start,
@@ -768,13 +793,41 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
     // case's constants in scalar registers (scalar loads) instead of vector loads in the loop
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     int longest = 0;
-    for (int g = blockIdx.x * 4 + wv; g < nslots; g += gridDim.x * 4) {
-        const int c = __builtin_amdgcn_readfirstlane(slot_case[g]);
-        if (flags[c] == 0 || g - cases[c].cl_base >= cl_count[c]) continue;     // wave-uniform
-        longest = max(longest, cluster_list_one<K>(cases[c], g, cl_info[g], pos, start, rank, width, cl_nbr, cl_len,
-                                                   selfin, count_out, max_len, cand[wv], candj[wv], candf[wv],
-                                                   srank[wv], clo[wv], cmid[wv], cpre[wv]));
+    const int4 none = make_int4(0, 0, 0, 0);
+#define MPH_LIST_ONE(PP, CC, G, I, G1, I1)                                                                  \
+    cluster_list_one<K, PP>(cases[CC], G, I, G1, I1, pos, start, rank, width, cl_nbr, cl_len, selfin,       \
+                            count_out, max_len, cand[wv], candj[wv], candf[wv], srank[wv], clo[wv], cmid[wv], \
+                            cpre[wv])
+    if (K <= 4) {
+        // slots in pairs (2t, 2t + 1): two clusters of one cell share a wavefront
+        for (int t = blockIdx.x * 4 + wv; 2 * t < nslots; t += gridDim.x * 4) {
+            const int g0 = 2 * t, g1 = g0 + 1;
+            const int c0 = __builtin_amdgcn_readfirstlane(slot_case[g0]);
+            const bool v0 = flags[c0] != 0 && g0 - cases[c0].cl_base < cl_count[c0];      // wave-uniform
+            int c1 = c0;
+            bool v1 = false;
+            if (g1 < nslots) {
+                c1 = __builtin_amdgcn_readfirstlane(slot_case[g1]);
+                v1 = flags[c1] != 0 && g1 - cases[c1].cl_base < cl_count[c1];
+            }
+            if (v0 && v1 && c0 == c1) {
+                const int4 i0 = cl_info[g0], i1 = cl_info[g1];
+                if (i0.z == i1.z) {
+                    longest = max(longest, MPH_LIST_ONE(2, c0, g0, i0, g1, i1));
+                    continue;
+                }
+            }
+            if (v0) longest = max(longest, MPH_LIST_ONE(1, c0, g0, cl_info[g0], 0, none));
+            if (v1) longest = max(longest, MPH_LIST_ONE(1, c1, g1, cl_info[g1], 0, none));
+        }
+    } else {
+        for (int g = blockIdx.x * 4 + wv; g < nslots; g += gridDim.x * 4) {
+            const int c = __builtin_amdgcn_readfirstlane(slot_case[g]);
+            if (flags[c] == 0 || g - cases[c].cl_base >= cl_count[c]) continue;     // wave-uniform
+            longest = max(longest, MPH_LIST_ONE(1, c, g, cl_info[g], 0, none));
+        }
     }
+#undef MPH_LIST_ONE
     // one atomic per wavefront: an atomic per cluster on the one counter serialised the
     // kernel (about 4 ms for 318k clusters)
     if ((threadIdx.x & 63) == 0 && longest > 0) atomicMax(max_len, longest);
