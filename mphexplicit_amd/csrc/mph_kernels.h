@@ -545,19 +545,19 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
     if (r == C.tcc - 1) cl_count[c] = base + nq;
 }
 
-// Union lists, one wavefront per cluster (main.cpp:1329-1596 per member, united in ascending
-// index).  The candidates are the particles of the cluster's 3^d stencil cells in ascending
-// rank and index.  The wavefront takes 64 at a time (one load per lane, the next block's
-// loads in flight), drops those farther than the cut-off from the members' bounding box
-// (conservatively: cut-off^2 (1 + 1e-9) against a box built from the same exact
-// differences), compacts the rest into LDS and tests them as K rows (members) x 64/K slots
-// with the reference's cut-off test.  A candidate joins the union if some member has it within
-// the cut-off and is not it; the union keeps candidate order.  Per member: NeighborCount as
-// the reference counts it (main.cpp:1408-1420).  Persistent wavefronts: a step without a
-// rebuild costs a small grid, and a rebuild of a few clusters runs them in parallel.  One
-// thread per member walking the candidates with a dependent load each measured 5.5-6 ms per
-// full rebuild of the 1.27M-particle deck (this kernel: 2.4-2.5 ms) and was latency-bound when
-// few clusters rebuild.
+// Union lists, one wavefront per cluster, or per two clusters of one cell when K <= 4
+// (main.cpp:1329-1596 per member, united in ascending index).  The candidates are the
+// particles of the cell's 3^d stencil cells in ascending rank and index.  The wavefront takes
+// 64 at a time (one load per lane, the next block's loads in flight), drops those farther than
+// the cut-off from the members' bounding box (conservatively: cut-off^2 (1 + 1e-9) against a
+// box built from the same exact differences), compacts the rest into LDS and tests them as
+// rows (members) x slots with the reference's cut-off test.  A candidate joins a cluster's
+// union if one of its members has it within the cut-off and is not it; the union keeps
+// candidate order.  Per member: NeighborCount as the reference counts it (main.cpp:1408-1420).
+// Wavefronts loop over the cluster slots: a step without a rebuild costs a small grid, and a
+// rebuild of a few clusters runs them in parallel.  One thread per member walking the
+// candidates with a dependent load each measured 5.5-6 ms per full rebuild of the
+// 1.27M-particle deck (this kernel: 1.9 ms) and was latency-bound when few clusters rebuild.
 
 // P clusters of one cell per wavefront (P = 2: rows 0..K-1 are the members of cluster g,
 // rows K..2K-1 those of cluster g1): the candidates of the shared stencil are loaded, located
