@@ -42,8 +42,6 @@ def build(verbose: bool = True) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "--offload-arch=" + ARCH, dev_obj, io_obj, "-o", tmp])
     os.replace(tmp, LIB)
-    _run([HIPCC, "-x", "hip"] + COMMON + DEVICE + [os.path.join(CSRC, "copy_probe.hip"), "-o",
-                                                 os.path.join(PKG, "copy_probe")])
     drv = os.path.join(CSRC, "mph_main.cpp")
     if os.path.exists(drv):
         _run(["g++"] + COMMON + [drv, "-o", DRIVER, "-L" + PKG, "-lmph_hip",

@@ -9,7 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
+#include <filesystem>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -109,12 +111,21 @@ struct mph_ctx {
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_src = nullptr;
     hipEvent_t ev_flag = nullptr;     // the eager step's flag readback has landed
+    hipEvent_t ev_cap = nullptr;      // the eager step's list-overflow readback has landed
 
-    ~mph_ctx()
+    void drop_graph()
     {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (graph_src) (void)hipGraphDestroy(graph_src);
+        graph = nullptr;
+        graph_src = nullptr;
+    }
+
+    ~mph_ctx()
+    {
+        drop_graph();
         if (ev_flag) (void)hipEventDestroy(ev_flag);
+        if (ev_cap) (void)hipEventDestroy(ev_cap);
         if (h_scalar) (void)hipHostFree(h_scalar);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -440,6 +451,24 @@ static void launch_scan(mph_ctx* c, unsigned* data, int total)
                        c->any_flag.p);
 }
 
+// the union lists of the cases flagged for a rebuild (count_only: only their lengths)
+static int launch_lists(mph_ctx* c, int count_only)
+{
+    // 4-wavefront blocks looping over the cluster slots.  A single case rebuilds all its
+    // clusters, and more resident rounds balance their uneven cost (318k clusters: 2.45 ms at
+    // 4096 blocks, 2.30 at 10240, 2.27 at 20480 = 16 rounds at 5 wavefronts per SIMD).  In a
+    // batch a rebuild usually covers a few cases, and the blocks that only skip slots cost
+    // more than the balance gains (64-case batch: rebuild phase 0.60 ms at 4096, 0.73 at
+    // 20480).  A step without a rebuild exits in about 5 microseconds either way.
+    const int lcap = c->ncases == 1 ? 20480 : 4096;
+    const int lgrid = std::max(1, std::min((c->nslots + 3) / 4, lcap));
+    LAUNCH_K(k_cluster_lists, c->K, dim3((unsigned)lgrid), dim3(256), 0, c->stream, c->d_cases.p,
+             c->slot_case.p, c->nslots, c->flags.p, c->any_flag.p, c->pos.p, c->start.p, c->rank.p,
+             c->cl_info.p, c->cl_count.p, count_only ? 0 : c->width, c->cl_nbr.p, c->cl_len.p, c->selfin.p, c->nout.p,
+             c->max_len.p);
+    return launch_ok("lists");
+}
+
 static int launch_rebuild(mph_ctx* c, int count_only)
 {
     const int nb = c->nblocks();
@@ -474,37 +503,28 @@ static int launch_rebuild(mph_ctx* c, int count_only)
     hipLaunchKernelGGL(k_cluster_fill, dim3(gcells), dim3(256), 0, s, c->d_cases.p, c->cell_case.p,
                        c->ncells, c->start.p, c->cell_of_rank.p, c->K, c->ccount.p, c->cl_info.p,
                        c->cl_count.p, c->any_flag.p);
-    // 4-wavefront blocks looping over the cluster slots.  A single case rebuilds all its
-    // clusters, and more resident rounds balance their uneven cost (318k clusters: 2.45 ms at
-    // 4096 blocks, 2.30 at 10240, 2.27 at 20480 = 16 rounds at 5 wavefronts per SIMD).  In a
-    // batch a rebuild usually covers a few cases, and the blocks that only skip slots cost
-    // more than the balance gains (64-case batch: rebuild phase 0.60 ms at 4096, 0.73 at
-    // 20480).  A step without a rebuild exits in about 5 microseconds either way.
-    const int lcap = c->ncases == 1 ? 20480 : 4096;
-    const int lgrid = std::max(1, std::min((c->nslots + 3) / 4, lcap));
-    LAUNCH_K(k_cluster_lists, c->K, dim3((unsigned)lgrid), dim3(256), 0, s, c->d_cases.p,
-             c->slot_case.p, c->nslots, c->flags.p, c->any_flag.p, c->pos.p, c->start.p, c->rank.p,
-             c->cl_info.p, c->cl_count.p, count_only ? 0 : c->width, c->cl_nbr.p, c->cl_len.p, c->selfin.p, c->nout.p,
-             c->max_len.p);
-    return launch_ok("rebuild");
+    return launch_lists(c, count_only);
 }
 
 // real: a step (advances wall centres); the autotuner launches with 0 so that repeated
-// launches leave the state unchanged.  count: count the interacting pairs (timed steps)
-static int launch_sweep_a(mph_ctx* c, int real, bool count = false)
+// launches leave the state unchanged.  count: count the interacting pairs (timed steps).
+// speculative: queued before the host knows whether the lists are rebuilt (returns at once
+// on the device if they are)
+static int launch_sweep_a(mph_ctx* c, int real, bool count = false, bool speculative = false)
 {
+    const int spec = speculative ? 1 : 0;
     if (c->mixed) {
         LAUNCH_SK(k_sweep_a_mixed, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream,
                   c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width,
                   c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p,
-                  real ? c->pairs.p : c->pairs_scratch.p, c->selfin.p);
+                  real ? c->pairs.p : c->pairs_scratch.p, c->selfin.p, c->any_flag.p, spec);
         return launch_ok("sweep A");
     }
     const int mode = (c->same_ap ? 1 : 0) | (c->ratio_one ? 2 : 0);
 #define SWEEP_A_ARGS                                                                                        \
     dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p,  \
         c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p,          \
-        c->vstrain.p, c->divp.p, real ? c->pairs.p : c->pairs_scratch.p, c->selfin.p
+        c->vstrain.p, c->divp.p, real ? c->pairs.p : c->pairs_scratch.p, c->selfin.p, c->any_flag.p, spec
     const bool cnt = count && real;
 #define SWEEP_A_MODE(M)                                                                            \
     do {                                                                                           \
@@ -563,37 +583,129 @@ static int read_int(mph_ctx* c, const int* dptr, int* out)
     return MPH_OK;
 }
 
+static int read_ll(mph_ctx* c, const long long* dptr, long long* out)
+{
+    long long* h = reinterpret_cast<long long*>(c->h_scalar) + 2;
+    HIP_TRY(hipMemcpyAsync(h, dptr, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *out = *h;
+    return MPH_OK;
+}
+
+// A list build of this step cut a list at the reserved width (any_flag[1]; every later
+// kernel of the step has returned at once).  Widen the table geometrically, to 1.25x the
+// longest list, carry the other cases' lists over, build the flagged cases' lists again and
+// clear the halt.  The reference sizes NeighborInd at every rebuild (main.cpp:1487-1495,
+// 1598-1602); here the table only grows.
+static int grow_lists(mph_ctx* c)
+{
+    for (int tries = 0; tries < 8; ++tries) {
+        int mx = 0;
+        int rc = read_int(c, c->max_len.p, &mx);
+        if (rc) return rc;
+        const int nw = std::max(c->width + 16, ((int)std::ceil(mx * 1.25) + 16 + 15) / 16 * 16);
+        DevBuf<int> wider;
+        if ((rc = wider.alloc((size_t)nw * c->nslots))) return rc;
+        const long long total = (long long)c->width * c->nslots;
+        const unsigned grid = (unsigned)std::max(1ll, std::min((total + 255) / 256, 65536ll));
+        hipLaunchKernelGGL(k_widen_lists, dim3(grid), dim3(256), 0, c->stream, c->cl_nbr.p, c->width, wider.p, nw,
+                           total);
+        if ((rc = launch_ok("widen lists"))) return rc;
+        std::swap(c->cl_nbr.p, wider.p);
+        std::swap(c->cl_nbr.n, wider.n);    // `wider` frees the old table (hipFree waits for the copy)
+        c->width = nw;
+        c->drop_graph();                    // captured launches hold the old table and width
+        HIP_TRY(hipMemsetAsync(c->any_flag.p + 1, 0, sizeof(int), c->stream));
+        HIP_TRY(hipMemsetAsync(c->max_len.p, 0, sizeof(int), c->stream));
+        if ((rc = launch_lists(c, 0))) return rc;
+        int halt = 0;
+        if ((rc = read_int(c, c->any_flag.p + 1, &halt))) return rc;
+        if (!halt) return MPH_OK;
+    }
+    return fail(MPH_ERR_CAPACITY, "the cluster neighbour lists could not be widened enough");
+}
+
+// finish a step whose list build overflowed: wider lists, then its sweeps
+static int finish_halted_step(mph_ctx* c, bool count = false)
+{
+    int rc = grow_lists(c);
+    if (!rc) rc = launch_sweep_a(c, 1, count);
+    if (!rc) rc = launch_sweep_b(c, 1);
+    return rc;
+}
+
+// launch_step, then finish it if its list build overflowed (synchronises)
+static int launch_step_checked(mph_ctx* c)
+{
+    int rc = launch_step(c);
+    int halt = 0;
+    if (!rc) rc = read_int(c, c->any_flag.p + 1, &halt);
+    if (!rc && halt) rc = finish_halted_step(c);
+    return rc;
+}
+
+// launch_rebuild, then widen the lists if it overflowed (synchronises)
+static int launch_rebuild_checked(mph_ctx* c)
+{
+    int rc = launch_rebuild(c, 0);
+    int halt = 0;
+    if (!rc) rc = read_int(c, c->any_flag.p + 1, &halt);
+    if (!rc && halt) rc = grow_lists(c);
+    return rc;
+}
+
+static void advance_clock(mph_ctx* c, int nsteps)
+{
+    for (int k = 0; k < nsteps; ++k)
+        for (auto& ch : c->cases) ch.time += ch.prm.dt;   // Time += Dt, main.cpp:449
+    c->steps += nsteps;
+}
+
 // The eager step decides the rebuild on the host, as the reference's loop does
 // (main.cpp:386-395): the Verlet flag of the move is read back and the 25 rebuild launches
 // are issued only when it is set.  Launched unconditionally they return at once without a
 // rebuild, but each costs about 4.4 us on the GPU (0.11 ms per step at 1.27M particles).
-// Sweep A is queued before the flag is known: it is right unless this step rebuilds the
-// lists, and then it is queued again after the rebuild (it has no side effect beyond its own
-// outputs), so the GPU does not wait for the host.  Graph replay keeps the device-side
+// Sweep A is queued before the flag is known (speculative): it runs unless this step
+// rebuilds the lists, when it returns at once on the device and is queued again after the
+// rebuild, so the GPU does not wait for the host.  After a rebuild the host reads whether a
+// list overflowed the width while the sweeps run, and if one did (the sweeps then returned
+// at once) widens the lists and finishes the step.  Graph replay keeps the device-side
 // decision.
+static int wait_event(hipEvent_t e)
+{
+    hipError_t q;
+    while ((q = hipEventQuery(e)) == hipErrorNotReady) {
+    }
+    return q == hipSuccess ? MPH_OK : fail(MPH_ERR_DEVICE, "step: %s", hipGetErrorString(q));
+}
+
 static int launch_step_host(mph_ctx* c)
 {
     int rc = launch_move(c);
     if (rc) return rc;
     HIP_TRY(hipMemcpyAsync(c->h_scalar, c->any_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipEventRecord(c->ev_flag, c->stream));
-    if ((rc = launch_sweep_a(c, 1))) return rc;
-    hipError_t q;
-    while ((q = hipEventQuery(c->ev_flag)) == hipErrorNotReady) {
-    }
-    if (q != hipSuccess) return fail(MPH_ERR_DEVICE, "step: %s", hipGetErrorString(q));
-    if (*c->h_scalar && ((rc = launch_rebuild(c, 0)) || (rc = launch_sweep_a(c, 1)))) return rc;
-    return launch_sweep_b(c, 1);
+    if ((rc = launch_sweep_a(c, 1, false, true))) return rc;
+    if ((rc = wait_event(c->ev_flag))) return rc;
+    if (!c->h_scalar[0]) return launch_sweep_b(c, 1);
+    if ((rc = launch_rebuild(c, 0))) return rc;
+    HIP_TRY(hipMemcpyAsync(c->h_scalar + 1, c->any_flag.p + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipEventRecord(c->ev_cap, c->stream));
+    if ((rc = launch_sweep_a(c, 1)) || (rc = launch_sweep_b(c, 1))) return rc;
+    if ((rc = wait_event(c->ev_cap))) return rc;
+    return c->h_scalar[1] ? finish_halted_step(c) : MPH_OK;
 }
 
+// end-of-call check: every overflow is resolved inside the step that had it, so a halt
+// left over here is an internal error
 static int check_capacity(mph_ctx* c)
 {
-    int mx = 0;
-    int rc = read_int(c, c->max_len.p, &mx);
+    int halt = 0;
+    int rc = read_int(c, c->any_flag.p + 1, &halt);
     if (rc) return rc;
-    if (mx > c->width)
-        return fail(MPH_ERR_CAPACITY, "a cluster neighbour list reached %d entries, above the reserved width %d; "
-                    "results of this call are invalid", mx, c->width);
+    if (halt)
+        return fail(MPH_ERR_CAPACITY, "a cluster neighbour list overflowed the reserved width %d and was not "
+                    "widened; results of this call are invalid", c->width);
     return MPH_OK;
 }
 
@@ -735,12 +847,18 @@ static int choose_cluster_size(mph_ctx* c, int* K)
     c->tune_v.release();
     c->tune_f.release();
     if (rc) return rc;
-    const std::string dir = path.substr(0, path.rfind('/'));
-    std::string mk = "mkdir -p '" + dir + "' 2>/dev/null";
-    (void)!std::system(mk.c_str());
-    if (FILE* f = std::fopen(path.c_str(), "a")) {
+    // the cache keeps later contexts of the same shape on the same size (sizes round
+    // differently); failing to write it is reported, and this context keeps its choice
+    const size_t cut = path.rfind('/');
+    std::error_code ec;
+    if (cut != std::string::npos && cut > 0) std::filesystem::create_directories(path.substr(0, cut), ec);
+    FILE* f = ec ? nullptr : std::fopen(path.c_str(), "a");
+    if (f) {
         std::fprintf(f, "%s %d\n", key.c_str(), *K);
         std::fclose(f);
+    } else {
+        std::fprintf(stderr, "mphexplicit_amd: cannot write the cluster-size cache %s (%s)\n", path.c_str(),
+                     ec ? ec.message().c_str() : std::strerror(errno));
     }
     return MPH_OK;
 }
@@ -754,6 +872,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     HIP_TRY(hipGetDevice(&c->device));
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&c->ev_flag, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&c->ev_cap, hipEventDisableTiming));
     HIP_TRY(hipHostMalloc((void**)&c->h_scalar, sizeof(long long) * 4, hipHostMallocDefault));
     c->ncases = ncases;
     c->cases.resize(ncases);
@@ -843,15 +962,16 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         (rc = c->vstrain.alloc(n)) || (rc = c->divp.alloc(n)) || (rc = c->vir.alloc(n)) ||
         (rc = c->wallc.alloc(hwall.size())) || (rc = c->keys_a.alloc(n)) || (rc = c->keys_b.alloc(n)) ||
         (rc = c->hist.alloc((size_t)kRadix * c->sort_blocks)) || (rc = c->perm_a.alloc(n)) ||
-        (rc = c->perm_b.alloc(n)) || (rc = c->flags.alloc(ncases)) || (rc = c->any_flag.alloc(1)) ||
+        (rc = c->perm_b.alloc(n)) || (rc = c->flags.alloc(ncases)) || (rc = c->any_flag.alloc(2)) ||
         (rc = c->rank.alloc(hrank.size())) || (rc = c->cell_of_rank.alloc(hcell.size())) ||
         (rc = c->cell_case.alloc(c->ncells)) || (rc = c->ccount.alloc(c->ncells)) ||
         (rc = c->scan_sums.alloc((size_t)std::max(kRadix * c->sort_blocks, c->ncells) / kScanTile + 2)) ||
         (rc = c->cl_count.alloc(ncases)) || (rc = c->max_len.alloc(1)) ||
-        (rc = c->rebuilds.alloc(ncases)) || (rc = c->pairs.alloc(kPairSlots * kPairStride)) ||
+        (rc = c->rebuilds.alloc(ncases + 1)) || (rc = c->pairs.alloc(kPairSlots * kPairStride)) ||
         (rc = c->pairs_scratch.alloc(kPairSlots * kPairStride)))
         return rc;
     hipStream_t s = c->stream;
+    HIP_TRY(hipMemsetAsync(c->any_flag.p, 0, 2 * sizeof(int), s));
     HIP_TRY(hipMemcpyAsync(c->d_bmap.p, c->bmap_host.data(), sizeof(int2) * c->bmap_host.size(), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->pos.p, hpos.data(), sizeof(double4) * n, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->vel.p, hvel.data(), sizeof(double4) * n, hipMemcpyHostToDevice, s));
@@ -863,7 +983,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     HIP_TRY(hipMemsetAsync(c->aux.p, 0, sizeof(double4) * n, s));
     HIP_TRY(hipMemsetAsync(c->force.p, 0, sizeof(double4) * n, s));
     HIP_TRY(hipMemsetAsync(c->vir.p, 0, sizeof(double) * n, s));
-    HIP_TRY(hipMemsetAsync(c->rebuilds.p, 0, sizeof(long long) * ncases, s));
+    HIP_TRY(hipMemsetAsync(c->rebuilds.p, 0, sizeof(long long) * (ncases + 1), s));
     HIP_TRY(hipMemsetAsync(c->pairs.p, 0, sizeof(unsigned long long) * kPairSlots * kPairStride, s));
     // first bucketing and lists (main.cpp:358-359), for the cluster size the autotuner
     // picks (it builds and times every size, the last build is then redone for the winner)
@@ -873,7 +993,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     if ((rc = launch_sweep_a(c, 0))) return rc;                          // main.cpp:362-364
     HIP_TRY(hipMemcpyAsync(c->vnext.p, c->vel.p, sizeof(double4) * n, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemsetAsync(c->flags.p, 0, sizeof(int) * ncases, s));
-    HIP_TRY(hipMemsetAsync(c->any_flag.p, 0, sizeof(int), s));
+    HIP_TRY(hipMemsetAsync(c->any_flag.p, 0, 2 * sizeof(int), s));
     std::vector<long long> one_ll(ncases, 1);
     HIP_TRY(hipMemcpyAsync(c->rebuilds.p, one_ll.data(), sizeof(long long) * ncases, hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -914,44 +1034,62 @@ extern "C" int mph_step(mph_ctx* c, int nsteps)
     for (int k = 0; k < nsteps; ++k) {
         int rc = launch_step_host(c);
         if (rc) return rc;
-        for (auto& ch : c->cases) ch.time += ch.prm.dt;   // Time += Dt, main.cpp:449
-        ++c->steps;
+        advance_clock(c, 1);
     }
     return check_capacity(c);
+}
+
+// Steps launched on the device-side decision (graph replays, timed steps) run on without the
+// host; a list overflow halts the rest of the call's launches on the device.  Afterwards the
+// completed-step counter (rebuilds[ncases], counted by sweep B) tells how many steps ran to
+// the end; the halted step is finished here and the caller launches the remaining ones.
+static int steps_done(mph_ctx* c, long long* out) { return read_ll(c, c->rebuilds.p + c->ncases, out); }
+
+static int resolve_halt(mph_ctx* c, long long done0, int launched, int* ran, bool count = false)
+{
+    int halt = 0, rc = read_int(c, c->any_flag.p + 1, &halt);
+    if (rc) return rc;
+    *ran = launched;
+    if (!halt) return MPH_OK;
+    long long done1 = 0;
+    if ((rc = steps_done(c, &done1))) return rc;
+    *ran = (int)(done1 - done0) + 1;
+    return finish_halted_step(c, count);
 }
 
 extern "C" int mph_step_graph(mph_ctx* c, int nsteps)
 {
     if (!c || nsteps < 0) return fail(MPH_ERR_ARG, "mph_step_graph: bad argument");
-    if (!c->graph) {
-        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = launch_step(c);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-        if (e != hipSuccess) return fail(MPH_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
-        c->graph_src = g;
-        HIP_TRY(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
-    }
-    for (int k = 0; k < nsteps; ++k) {
-        HIP_TRY(hipGraphLaunch(c->graph, c->stream));
-        for (auto& ch : c->cases) ch.time += ch.prm.dt;
-        ++c->steps;
+    int left = nsteps;
+    while (left > 0) {
+        if (!c->graph) {
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int rc = launch_step(c);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (e != hipSuccess) return fail(MPH_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+            c->graph_src = g;
+            HIP_TRY(hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+        }
+        long long done0 = 0;
+        int rc = steps_done(c, &done0), ran = 0;
+        if (rc) return rc;
+        for (int k = 0; k < left; ++k) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+        if ((rc = resolve_halt(c, done0, left, &ran))) return rc;
+        advance_clock(c, ran);
+        left -= ran;
     }
     return check_capacity(c);
 }
 
-extern "C" int mph_step_timed(mph_ctx* c, int nsteps, mph_timing* out)
+static int step_timed_once(mph_ctx* c, int nsteps, mph_timing* out, int* ran)
 {
-    if (!c || nsteps < 0 || !out) return fail(MPH_ERR_ARG, "mph_step_timed: bad argument");
-    std::memset(out, 0, sizeof(*out));
-    std::vector<long long> rb0(c->ncases), rb1(c->ncases);
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemcpy(rb0.data(), c->rebuilds.p, sizeof(long long) * c->ncases, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemsetAsync(c->pairs.p, 0, sizeof(unsigned long long) * kPairSlots * kPairStride, c->stream));
+    long long done0 = 0;
+    int rc = steps_done(c, &done0);
+    if (rc) return rc;
     std::vector<hipEvent_t> ev((size_t)nsteps * 5, nullptr);
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-    int rc = MPH_OK;
     for (int k = 0; k < nsteps && rc == MPH_OK; ++k) {
         hipEvent_t* e = &ev[(size_t)k * 5];
         (void)hipEventRecord(e[0], c->stream);
@@ -963,8 +1101,6 @@ extern "C" int mph_step_timed(mph_ctx* c, int nsteps, mph_timing* out)
         (void)hipEventRecord(e[3], c->stream);
         if (!rc) rc = launch_sweep_b(c, 1);
         (void)hipEventRecord(e[4], c->stream);
-        for (auto& ch : c->cases) ch.time += ch.prm.dt;
-        ++c->steps;
     }
     hipError_t se = hipStreamSynchronize(c->stream);
     double* slot[4] = {&out->move_ms, &out->rebuild_ms, &out->sweep_a_ms, &out->sweep_b_ms};
@@ -977,6 +1113,23 @@ extern "C" int mph_step_timed(mph_ctx* c, int nsteps, mph_timing* out)
     for (auto& e : ev) if (e) (void)hipEventDestroy(e);
     if (rc) return rc;
     if (se != hipSuccess) return fail(MPH_ERR_DEVICE, "step failed: %s", hipGetErrorString(se));
+    return resolve_halt(c, done0, nsteps, ran, true);
+}
+
+extern "C" int mph_step_timed(mph_ctx* c, int nsteps, mph_timing* out)
+{
+    if (!c || nsteps < 0 || !out) return fail(MPH_ERR_ARG, "mph_step_timed: bad argument");
+    std::memset(out, 0, sizeof(*out));
+    std::vector<long long> rb0(c->ncases), rb1(c->ncases);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(rb0.data(), c->rebuilds.p, sizeof(long long) * c->ncases, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemsetAsync(c->pairs.p, 0, sizeof(unsigned long long) * kPairSlots * kPairStride, c->stream));
+    for (int left = nsteps; left > 0;) {
+        int ran = 0, rc = step_timed_once(c, left, out, &ran);
+        if (rc) return rc;
+        advance_clock(c, ran);
+        left -= ran;
+    }
     std::vector<unsigned long long> slots((size_t)kPairSlots * kPairStride);
     HIP_TRY(hipMemcpy(slots.data(), c->pairs.p, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost));
     unsigned long long pairs = 0;
@@ -1254,6 +1407,11 @@ static int sur_pack(mph_surrogate* s)
 static int sur_pairs(mph_surrogate* s)
 {
     mph_ctx* c = s->c;
+    if (c->width > s->wh) {        // the lists were widened since: so is the pair table
+        int rc = s->pnbr.alloc((size_t)c->n * c->width);
+        if (rc) return rc;
+        s->wh = c->width;
+    }
     const long threads = (long)c->nslots * c->K;
     hipLaunchKernelGGL(k_sur_pairs, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream, c->d_cases.p,
                        c->slot_case.p, c->nslots, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width,
@@ -1293,13 +1451,14 @@ static int sur_kick(mph_surrogate* s)
 }
 
 // one surrogate step (2 dt): move, lists, densities, prediction, kick, move, lists, kick
+// (a list overflow is resolved at once: the rebuilds are checked on the host)
 static int sur_step(mph_surrogate* s)
 {
     mph_ctx* c = s->c;
     int rc = 0;
-    if ((rc = launch_move(c)) || (rc = launch_rebuild(c, 0)) || (rc = launch_sweep_a(c, 1)) ||
-        (rc = sur_infer(s, c->vel.p)) || (rc = sur_kick(s)) || (rc = launch_move(c)) || (rc = launch_rebuild(c, 0)) ||
-        (rc = sur_kick(s)))
+    if ((rc = launch_move(c)) || (rc = launch_rebuild_checked(c)) || (rc = launch_sweep_a(c, 1)) ||
+        (rc = sur_infer(s, c->vel.p)) || (rc = sur_kick(s)) || (rc = launch_move(c)) ||
+        (rc = launch_rebuild_checked(c)) || (rc = sur_kick(s)))
         return rc;
     for (auto& ch : c->cases) ch.time += 2.0 * ch.prm.dt;
     c->steps += 2;
@@ -1584,7 +1743,7 @@ extern "C" int mph_sur_train(mph_surrogate* s, int samples, int iters, double* l
         return rc;
     for (int smp = 0; smp < samples; ++smp) {
         // plain step n+1: state (x_{n+1}, v_n) and volumetric strain at x_{n+1}
-        if ((rc = launch_step(c))) return rc;
+        if ((rc = launch_step_checked(c))) return rc;
         for (auto& ch : c->cases) ch.time += ch.prm.dt;
         ++c->steps;
         if ((rc = sur_pairs(s)) || (rc = sur_make_list(s))) return rc;
@@ -1593,7 +1752,7 @@ extern "C" int mph_sur_train(mph_surrogate* s, int samples, int iters, double* l
         HIP_TRY(hipMemcpyAsync(s->sdens.p, c->vstrain.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
         HIP_TRY(hipMemcpyAsync(s->spidx.p, c->pidx.p, n * sizeof(int), hipMemcpyDeviceToDevice, st));
         // plain step n+2: v_{n+2}, by original id
-        if ((rc = launch_step(c))) return rc;
+        if ((rc = launch_step_checked(c))) return rc;
         for (auto& ch : c->cases) ch.time += ch.prm.dt;
         ++c->steps;
         hipLaunchKernelGGL(k_sur_to_orig, dim3((c->n + 255) / 256), dim3(256), 0, st, c->n, c->pidx.p, c->vnext.p,

@@ -103,14 +103,19 @@ __device__ __forceinline__ double image_near(double x, double xf, double hw, dou
 // the reference's Mod form only by the rounding of the +-W/2 round trip.  Discrete
 // decisions (list cut-off, cell keys) keep the reference's form (rel3).
 
-// 1/sqrt(q2) for the pair sweeps: the hardware estimate (measured on MI355X: at most
-// 5.2e-8 relative error) and one Newton step, at most 4.1e-15 relative (37 ulp) -- far
-// inside the FP64 field tolerance and three FP64 instructions per pair cheaper
+// 1/sqrt(q2) for the pair sweeps: the hardware estimate y (measured on MI355X: at most
+// 5.2e-8 relative error) and one third-order correction.  With e = 1 - q2 y^2,
+//   1/sqrt(q2) = y (1 - e)^(-1/2) = y (1 + e/2 + 3e^2/8 + O(e^3)),
+// so the truncation error is below 0.3 |e|^3 ~ 4e-23 and the result is within a few ulp
+// (rounding of e, of the polynomial and of the final fma): one instruction more than a
+// Newton step, whose 1.5 e^2 ~ 4e-15 (37 ulp) error the pressure of the 1M-particle dam
+// break amplified to 1.9e-12 of the field (P is a difference of sums near N0p).
 __device__ __forceinline__ double rsqrt_pair(double q2)
 {
     const double y = __builtin_amdgcn_rsq(q2);
-    const double e = fma(-(0.5 * q2) * y, y, 0.5);
-    return fma(y, e, y);
+    const double e = fma(-(q2 * y), y, 1.0);
+    const double t = e * fma(e, 0.375, 0.5);
+    return fma(y, t, y);
 }
 
 // The pair sweeps evaluate every union entry without a branch.  Padding entries carry a
@@ -121,6 +126,16 @@ __device__ __forceinline__ double rsqrt_pair(double q2)
 // the same position would make the reference divide by zero; the sweeps stay finite.
 constexpr double kFar = 1.0e30;
 __device__ __forceinline__ double far_if_self(double q2) { return q2 > 0.0 ? q2 : 1.0e300; }
+
+// The step's device flags, any_flag[0..1]:
+//   [0] some case rebuilds its lists this step (the Verlet trigger, raised by the move)
+//   [1] a list build of this step outgrew the reserved list width.  Every later kernel then
+//       returns at once, so the state stays that of the overflowing step after its rebuild
+//       (positions sorted, lists incomplete, rebuild flags still set); the host widens the
+//       lists, builds them again and finishes the step (the reference sizes NeighborInd at
+//       every rebuild, main.cpp:1487-1495, and never runs out of room).
+__device__ __forceinline__ bool halted(const int* any_flag) { return any_flag[1] != 0; }
+__device__ __forceinline__ bool no_rebuild(const int* any_flag) { return any_flag[0] == 0 || any_flag[1] != 0; }
 
 
 // -------------------------------------------------------------------------------------
@@ -133,6 +148,7 @@ k_move(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
        const double4* __restrict__ snap, const double* __restrict__ wallc,
        int* __restrict__ flags, int* __restrict__ any_flag)
 {
+    if (halted(any_flag)) return;
     const int2 m = bmap[blockIdx.x];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
@@ -224,7 +240,7 @@ k_keys(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
        const int* __restrict__ any_flag, const int* __restrict__ rank, const unsigned* kprev,
        unsigned* kout, int* vout)
 {
-    if (*any_flag == 0) return;
+    if (no_rebuild(any_flag)) return;
     const int2 m = bmap[blockIdx.x];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
@@ -245,7 +261,7 @@ __global__ void __launch_bounds__(kSortThreads)
 k_radix_hist(const unsigned* __restrict__ keys, int n, int shift, unsigned* __restrict__ hist,
              const int* __restrict__ any_flag)
 {
-    if (*any_flag == 0) return;
+    if (no_rebuild(any_flag)) return;
     __shared__ unsigned cnt[kRadix];
     const int tid = threadIdx.x;
     cnt[tid] = 0;
@@ -263,7 +279,7 @@ k_radix_hist(const unsigned* __restrict__ keys, int n, int shift, unsigned* __re
 __global__ void __launch_bounds__(1024)
 k_scan_exclusive(unsigned* __restrict__ data, int total, const int* __restrict__ any_flag)
 {
-    if (any_flag && *any_flag == 0) return;
+    if (any_flag && no_rebuild(any_flag)) return;
     __shared__ unsigned part[1024];
     const int tid = threadIdx.x;
     const int chunk = (total + 1023) / 1024;
@@ -297,7 +313,7 @@ __global__ void __launch_bounds__(kScanThreads)
 k_scan_tile_sums(const unsigned* __restrict__ data, int total, unsigned* __restrict__ sums,
                  const int* __restrict__ any_flag)
 {
-    if (any_flag && *any_flag == 0) return;
+    if (any_flag && no_rebuild(any_flag)) return;
     const int base = blockIdx.x * kScanTile;
     unsigned s = 0;
     for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) s += base + k < total ? data[base + k] : 0u;
@@ -316,7 +332,7 @@ __global__ void __launch_bounds__(kScanThreads)
 k_scan_tiles(unsigned* __restrict__ data, int total, const unsigned* __restrict__ offsets,
              const int* __restrict__ any_flag)
 {
-    if (any_flag && *any_flag == 0) return;
+    if (any_flag && no_rebuild(any_flag)) return;
     // padded so that the 16 consecutive counters of one thread fall in different banks
     __shared__ unsigned tile[kScanTile + kScanTile / kScanItems];
     __shared__ unsigned part[kScanThreads];
@@ -354,7 +370,7 @@ k_radix_scatter(const unsigned* __restrict__ kin, const int* __restrict__ vin,
                 unsigned* __restrict__ kout, int* __restrict__ vout, int n, int shift,
                 const unsigned* __restrict__ hist, const int* __restrict__ any_flag)
 {
-    if (*any_flag == 0) return;
+    if (no_rebuild(any_flag)) return;
     __shared__ unsigned base[kRadix];
     __shared__ unsigned wcnt[kSortThreads / 64][kRadix];
     const int tid = threadIdx.x;
@@ -399,7 +415,7 @@ __global__ void __launch_bounds__(256)
 k_cell_start(const unsigned* __restrict__ keys, int n, int nkeys, int* __restrict__ start,
              const int* __restrict__ any_flag)
 {
-    if (*any_flag == 0) return;
+    if (no_rebuild(any_flag)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k > nkeys) return;
     const unsigned kk = (unsigned)k << kSubBits;
@@ -420,7 +436,7 @@ k_gather(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
          const double4* __restrict__ vel, const int* __restrict__ pidx,
          double4* __restrict__ pos_alt, double4* __restrict__ vel_alt, int* __restrict__ pidx_alt)
 {
-    if (*any_flag == 0) return;
+    if (no_rebuild(any_flag)) return;
     const int2 m = bmap[blockIdx.x];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
@@ -439,7 +455,7 @@ k_scatter_back(const CaseConst* __restrict__ cases, const int2* __restrict__ bma
                const int* __restrict__ pidx_alt, double4* __restrict__ pos,
                double4* __restrict__ vel, int* __restrict__ pidx, double4* __restrict__ snap)
 {
-    if (*any_flag == 0) return;
+    if (no_rebuild(any_flag)) return;
     const int2 m = bmap[blockIdx.x];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
@@ -511,7 +527,7 @@ k_cluster_count(const CaseConst* __restrict__ cases, const int* __restrict__ cel
                 const int* __restrict__ start, int K, unsigned* __restrict__ ccount,
                 const int* __restrict__ any_flag)
 {
-    if (*any_flag == 0) return;
+    if (no_rebuild(any_flag)) return;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ncells) return;
     const CaseConst& C = cases[cell_case[s]];
@@ -527,7 +543,7 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
                const unsigned* __restrict__ coff, int4* __restrict__ cl_info, int* __restrict__ cl_count,
                const int* __restrict__ any_flag)
 {
-    if (*any_flag == 0) return;
+    if (no_rebuild(any_flag)) return;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ncells) return;
     const int c = cell_case[s];
@@ -777,14 +793,14 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
 template <int K>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slot_case, int nslots,
-                const int* __restrict__ flags, const int* __restrict__ any_flag,
+                const int* __restrict__ flags, int* __restrict__ any_flag,
                 const double4* __restrict__ pos, const int* __restrict__ start,
                 const int* __restrict__ rank, const int4* __restrict__ cl_info,
                 const int* __restrict__ cl_count, int width, int* __restrict__ cl_nbr,
                 int* __restrict__ cl_len, int* __restrict__ selfin, int* __restrict__ count_out,
                 int* __restrict__ max_len)
 {
-    if (*any_flag == 0) return;
+    if (no_rebuild(any_flag)) return;
     __shared__ double4 cand[4][64];
     __shared__ int candj[4][64];         // particle index
     __shared__ int candf[4][64];         // 1: in the fluid block of its cell
@@ -830,7 +846,24 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
 #undef MPH_LIST_ONE
     // one atomic per wavefront: an atomic per cluster on the one counter serialised the
     // kernel (about 4 ms for 318k clusters)
-    if ((threadIdx.x & 63) == 0 && longest > 0) atomicMax(max_len, longest);
+    if ((threadIdx.x & 63) == 0 && longest > 0) {
+        atomicMax(max_len, longest);
+        // a list longer than the width was cut: the rest of the step waits for the host
+        // (width 0 is the counting pass that sizes the lists)
+        if (width > 0 && longest > width) atomicOr(any_flag + 1, 1);
+    }
+}
+
+// copy every cluster list into a wider table (the lists of cases that did not rebuild stay
+// valid; the rebuilt ones are built again into the new table)
+__global__ void __launch_bounds__(256)
+k_widen_lists(const int* __restrict__ src, int w_src, int* __restrict__ dst, int w_dst, long long total)
+{
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long g = e / w_src;
+        dst[g * w_dst + (e - g * w_src)] = src[e];
+    }
 }
 
 // -------------------------------------------------------------------------------------
@@ -1134,9 +1167,13 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int* __restrict__ count_out,
           const double4* __restrict__ pos, double4* vel, double4* __restrict__ aux,
           double* __restrict__ dens_a, double* __restrict__ vstrain, double* __restrict__ divp,
-          unsigned long long* __restrict__ pair_count, const int* __restrict__ selfin)
+          unsigned long long* __restrict__ pair_count, const int* __restrict__ selfin,
+          const int* __restrict__ any_flag, int speculative)
 {
-    // no side effect beyond its outputs: the eager step may run it once more after a rebuild
+    // no side effect beyond its outputs: the eager step queues it before it knows whether the
+    // lists are rebuilt (speculative) and, if they are, once more after the rebuild; the
+    // speculative launch returns at once when the move raised the rebuild flag
+    if (halted(any_flag) || (speculative && any_flag[0] != 0)) return;
     const int2 m = cmap[blockIdx.x];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
@@ -1315,6 +1352,7 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           double4* __restrict__ force, int* __restrict__ flags,
           int* __restrict__ any_flag, long long* __restrict__ rebuilds, double* __restrict__ wallc, int ncases)
 {
+    if (halted(any_flag)) return;
     if (blockIdx.x == 0 && threadIdx.x == 0 && ncases > 0) {
         for (int c = 0; c < ncases; ++c) {
             if (flags[c]) rebuilds[c] += 1;
@@ -1325,6 +1363,7 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
                 for (int d = 0; d < 3; ++d)
                     wallc[(c * kTypes + t) * 3 + d] += cases[c].wvel[t][d] * cases[c].dt;
         }
+        rebuilds[ncases] += 1;          // completed steps
         *any_flag = 0;
     }
     const int2 m = cmap[blockIdx.x];

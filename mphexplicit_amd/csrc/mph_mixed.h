@@ -165,8 +165,11 @@ k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
                 const int* __restrict__ count_out,
                 const double4* __restrict__ pos, double4* vel, double4* __restrict__ aux,
                 double* __restrict__ dens_a, double* __restrict__ vstrain, double* __restrict__ divp,
-                unsigned long long* __restrict__ pair_count, const int* __restrict__ selfin)
+                unsigned long long* __restrict__ pair_count, const int* __restrict__ selfin,
+                const int* __restrict__ any_flag, int speculative)
 {
+    // as k_sweep_a: a speculative launch returns at once when the lists are being rebuilt
+    if (halted(any_flag) || (speculative && any_flag[0] != 0)) return;
     const int2 m = cmap[blockIdx.x];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
@@ -349,6 +352,7 @@ k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
                 int* __restrict__ any_flag, long long* __restrict__ rebuilds, double* __restrict__ wallc,
                 int ncases)
 {
+    if (halted(any_flag)) return;
     if (blockIdx.x == 0 && threadIdx.x == 0 && ncases > 0) {
         for (int c = 0; c < ncases; ++c) {
             if (flags[c]) rebuilds[c] += 1;
@@ -358,6 +362,7 @@ k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
                 for (int d = 0; d < 3; ++d)
                     wallc[(c * kTypes + t) * 3 + d] += cases[c].wvel[t][d] * cases[c].dt;
         }
+        rebuilds[ncases] += 1;          // completed steps
         *any_flag = 0;
     }
     const int2 m = cmap[blockIdx.x];

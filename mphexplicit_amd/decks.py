@@ -369,6 +369,26 @@ def sparse_random_3d(n=1500, side=24, l0=1.0e-3, seed=17, steps=10, name=None) -
                 v, steps=steps)
 
 
+def collapsing_cloud_3d(n_side=14, spread=1.45, l0=1.0e-3, vmax=0.2, seed=18, steps=200, name=None) -> Deck:
+    """A sparse cubic cloud (lattice spacing ``spread``*l0, about a third of the fluid
+    density) falling in on its centre, v = -alpha x with the corners at ``vmax``, in a
+    periodic box without gravity.  Sound speed 10 vmax, dt from CFL 0.2.  After ``steps``
+    = 200 steps the cloud has shrunk by about a quarter and its neighbour lists have more
+    than doubled, past the width reserved at the first build."""
+    rng = np.random.default_rng(seed)
+    rho, mu = 1000.0, 1.0e-3
+    half = 0.5 * n_side * spread * l0
+    x = _jitter(rng, _lattice((n_side,) * 3, spread * l0, (-half, -half, -half)), 0.01 * l0, 3)
+    alpha = vmax / float(np.max(np.linalg.norm(x, axis=1)))
+    v = -alpha * x
+    K, dt = sound_speed_params(rho, 10.0 * vmax, l0)
+    L = half + 6.0 * l0
+    data = DataParams(dt=dt, end_time=dt * steps, density=(rho,) * 4, bulk_modulus=(K,) * 4,
+                      bulk_viscosity=(mu,) * 4, shear_viscosity=(mu,) * 4, surface_tension=(0.0,) * 4)
+    return Deck(name or "collapsing_cloud3d", 3, data, l0, np.full(3, -L), np.full(3, L),
+                np.zeros(len(x), np.int32), x, v, steps=steps)
+
+
 def drift_dam_break_3d(seed=14) -> Deck:
     """Small 3D dam break for 1000-step runs: 8x11x8 fluid in a 14x11x12 walled box."""
     return dam_break_3d(fluid=(8, 11, 8), box=(14, 11, 12), jitter=0.01, seed=seed, steps=1000,

@@ -710,6 +710,33 @@ void orc_neighbors(const orc_state *s, long *ptr, int *ind)
 }
 
 /* calculateVirialStressAtParticle, main.cpp:2057-2302 */
+/* Not part of the restatement: VolStrainP of main.cpp:1831-1858 over the same lists and
+ * cut-off decisions, with the weights and their sum in long double (x87, 64-bit
+ * significand), so within about 1e-18 of the exact value for the same double inputs.  The
+ * tests use it to measure how far the FP64 sums of the reference and of the GPU each are
+ * from the exact sum, where VolStrainP - a difference of sums near N0p - makes the
+ * pressure sensitive to summation order. */
+void orc_vol_strain_exact(const orc_state *s, double *out)
+{
+    const long double c = (1.0L / (long double)s->swp) * (1.0L / (long double)hd(s, s->hp));
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < s->n; ++i) {
+        long double sum = 0.0L;
+        FOR_NEIGHBORS(s, i, j) {
+            double q[3];
+            rel(s, s->x[i], s->x[j], q);
+            const double r2 = (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+            if (s->hp * s->hp - r2 >= 0) {
+                const long double rl = sqrtl((long double)q[0] * q[0] + (long double)q[1] * q[1] +
+                                             (long double)q[2] * q[2]);
+                const long double om = 1.0L - rl / (long double)s->hp;
+                sum += c * (om * om);
+            }
+        }
+        out[i] = (double)(sum - (long double)s->n0p);
+    }
+}
+
 void orc_virial(orc_state *s, double *vp)
 {
     const double cd = s->dim == 2 ? 8.0 : 10.0;

@@ -33,6 +33,9 @@ void orc_get(const orc_state *s, int *index, int *type, double *x, double *v, do
              double *press_a, double *gcenter, int *ncount, double *mass);
 void orc_neighbors(const orc_state *s, long *ptr, int *ind);
 void orc_virial(orc_state *s, double *vp);
+/* VolStrainP with long double weights and sums (not part of the restatement: the exact
+ * value that the tests measure both codes' FP64 rounding against) */
+void orc_vol_strain_exact(const orc_state *s, double *out);
 void orc_destroy(orc_state *s);
 
 #ifdef __cplusplus

@@ -91,6 +91,7 @@ class Oracle:
         lib.orc_get.argtypes = [C.c_void_p] + [C.c_void_p] * len(FIELDS)
         lib.orc_neighbors.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         lib.orc_virial.argtypes = [C.c_void_p, C.c_void_p]
+        lib.orc_vol_strain_exact.argtypes = [C.c_void_p, C.c_void_p]
         lib.orc_scalars.argtypes = [C.c_void_p, C.c_void_p]
         lib.orc_destroy.argtypes = [C.c_void_p]
         self.lib = lib
@@ -135,6 +136,13 @@ class Oracle:
         o = np.zeros(16)
         self.lib.orc_scalars(self.h, o.ctypes.data)
         return o
+
+    def vol_strain_exact(self) -> np.ndarray:
+        """VolStrainP (current order) with long double sums: the exact value for the same
+        inputs, against which both codes' FP64 summation rounding is measured"""
+        out = np.zeros(self.n)
+        self.lib.orc_vol_strain_exact(self.h, out.ctypes.data)
+        return out
 
     def close(self):
         if self.h:

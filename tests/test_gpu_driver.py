@@ -27,7 +27,9 @@ def _run(tmp, extra=()):
     dpath, gpath = d.write(str(tmp), "deck")
     cmd = [DRIVER, dpath, gpath, "prof%03d.prof", "vtk%03d.vtk", "log.txt", "4", "-2d", *extra]
     print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, cwd=str(tmp), capture_output=True, text=True, timeout=120)
+    # the eager and graph runs are compared bit for bit: pin the cluster size
+    env = dict(os.environ, MPH_TUNE="4")
+    r = subprocess.run(cmd, cwd=str(tmp), capture_output=True, text=True, timeout=120, env=env)
     assert r.returncode == 0, r.stderr[-2000:]
     return d
 

@@ -260,3 +260,37 @@ def test_planted_sign_error_is_caught():
     k = int(np.argmax(np.abs(bad[:, 0])))
     bad[k] = -bad[k]
     assert not np.array_equal(bad, g["step1_force"])
+
+
+def test_vol_strain_exact_against_fsum():
+    # the long double VolStrainP (orc_vol_strain_exact) equals an independent correctly
+    # rounded sum (math.fsum over the weights evaluated in Python's exact-rounding sqrt and
+    # FP64 polynomial) to far below FP64 rounding of a sum near N0p, and the oracle's own
+    # FP64 sum is within that rounding
+    d = decks.small_dam_break_3d()
+    o = Oracle(d, nthreads=4)
+    o.step(1)
+    f = o.fields()
+    exact = o.vol_strain_exact()
+    sc = o.scalars()
+    n0p = sc[1]
+    ptr, ind = o.neighbors()
+    x = f["x"]
+    lo, hi = d.domain_min, d.domain_max
+    w = hi - lo
+    hp = d.data.radius_ratio_p * d.spacing
+    swp = 2.0 * math.pi / (15.0 * d.spacing ** 3)      # 3D S_P, main.cpp:853-857
+    rng = np.random.default_rng(3)
+    for i in rng.choice(d.n, size=200, replace=False):
+        terms = []
+        for j in ind[ptr[i]:ptr[i + 1]]:
+            if j == i:
+                continue
+            q = np.mod(x[j] - x[i] + 0.5 * w, w) - 0.5 * w
+            r2 = float(q[0] * q[0] + q[1] * q[1] + q[2] * q[2])
+            if hp * hp - r2 >= 0:
+                om = 1.0 - math.sqrt(r2) / hp
+                terms.append(om * om / (swp * hp ** 3))
+        ref = math.fsum(terms + [-n0p])
+        assert abs(exact[i] - ref) <= 4e-16 * n0p, (i, exact[i], ref)
+        assert abs(f["vol_strain"][i] - exact[i]) <= 64 * 1.2e-16 * n0p
