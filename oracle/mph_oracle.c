@@ -711,14 +711,17 @@ void orc_neighbors(const orc_state *s, long *ptr, int *ind)
 
 /* calculateVirialStressAtParticle, main.cpp:2057-2302 */
 /* Not part of the restatement: VolStrainP of main.cpp:1831-1858 over the same lists and
- * cut-off decisions, with the weights and their sum in long double (x87, 64-bit
- * significand), so within about 1e-18 of the exact value for the same double inputs.  The
- * tests use it to measure how far the FP64 sums of the reference and of the GPU each are
- * from the exact sum, where VolStrainP - a difference of sums near N0p - makes the
- * pressure sensitive to summation order. */
+ * cut-off decisions, evaluated exactly up to about 1e-18: the pair differences as the
+ * exact minimum image x_j - x_i (the reference's Mod(x_j - x_i + W/2, W) - W/2 rounds in
+ * the +-W/2 round trip), the weights and their sum in long double (x87, 64-bit
+ * significand).  The tests use it to measure how far the FP64 VolStrainP of the reference
+ * and of the GPU each are from the exact value for the same positions: as a difference of
+ * sums near N0p it makes the pressure sensitive to every rounding of the pair geometry and
+ * of the summation order. */
 void orc_vol_strain_exact(const orc_state *s, double *out)
 {
-    const long double c = (1.0L / (long double)s->swp) * (1.0L / (long double)hd(s, s->hp));
+    /* the weight's constant factor as the reference forms it in double (main.cpp:214-284) */
+    const long double c = (long double)(1.0 / s->swp * 1.0 / hd(s, s->hp));
     #pragma omp parallel for schedule(static)
     for (int i = 0; i < s->n; ++i) {
         long double sum = 0.0L;
@@ -726,10 +729,16 @@ void orc_vol_strain_exact(const orc_state *s, double *out)
             double q[3];
             rel(s, s->x[i], s->x[j], q);
             const double r2 = (q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
-            if (s->hp * s->hp - r2 >= 0) {
-                const long double rl = sqrtl((long double)q[0] * q[0] + (long double)q[1] * q[1] +
-                                             (long double)q[2] * q[2]);
-                const long double om = 1.0L - rl / (long double)s->hp;
+            if (i != j && s->hp * s->hp - r2 >= 0) {
+                long double e2 = 0.0L;
+                for (int d = 0; d < 3; ++d) {
+                    const long double w = (long double)s->dwid[d];
+                    long double a = (long double)s->x[j][d] - (long double)s->x[i][d];
+                    if (a > 0.5L * w) a -= w;
+                    else if (a < -0.5L * w) a += w;
+                    e2 += a * a;
+                }
+                const long double om = 1.0L - sqrtl(e2) / (long double)s->hp;
                 sum += c * (om * om);
             }
         }

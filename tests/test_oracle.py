@@ -262,11 +262,14 @@ def test_planted_sign_error_is_caught():
     assert not np.array_equal(bad, g["step1_force"])
 
 
-def test_vol_strain_exact_against_fsum():
-    # the long double VolStrainP (orc_vol_strain_exact) equals an independent correctly
-    # rounded sum (math.fsum over the weights evaluated in Python's exact-rounding sqrt and
-    # FP64 polynomial) to far below FP64 rounding of a sum near N0p, and the oracle's own
-    # FP64 sum is within that rounding
+def test_vol_strain_exact_against_decimal():
+    # the exact VolStrainP (orc_vol_strain_exact: exact minimum-image differences, long
+    # double weights and sum) equals a 40-digit decimal evaluation of the same terms far below
+    # FP64 rounding of a sum near N0p; the oracle's own FP64 VolStrainP is within its rounding
+    import decimal
+    from fractions import Fraction
+    decimal.getcontext().prec = 40
+    D = decimal.Decimal
     d = decks.small_dam_break_3d()
     o = Oracle(d, nthreads=4)
     o.step(1)
@@ -276,21 +279,32 @@ def test_vol_strain_exact_against_fsum():
     n0p = sc[1]
     ptr, ind = o.neighbors()
     x = f["x"]
-    lo, hi = d.domain_min, d.domain_max
-    w = hi - lo
+    w = d.domain_max - d.domain_min
     hp = d.data.radius_ratio_p * d.spacing
-    swp = 2.0 * math.pi / (15.0 * d.spacing ** 3)      # 3D S_P, main.cpp:853-857
+    # S_P equals S_G in 3D (main.cpp:853-857); the weight's factor as the reference forms it
+    swp = sc[11]
+    assert swp == pytest.approx(2.0 * math.pi / (15.0 * d.spacing ** 3), rel=1e-15)
+    c = D(1.0 / swp * 1.0 / (hp * hp * hp))
     rng = np.random.default_rng(3)
-    for i in rng.choice(d.n, size=200, replace=False):
-        terms = []
+    for i in rng.choice(d.n, size=100, replace=False):
+        acc = D(0)
         for j in ind[ptr[i]:ptr[i + 1]]:
             if j == i:
                 continue
-            q = np.mod(x[j] - x[i] + 0.5 * w, w) - 0.5 * w
-            r2 = float(q[0] * q[0] + q[1] * q[1] + q[2] * q[2])
-            if hp * hp - r2 >= 0:
-                om = 1.0 - math.sqrt(r2) / hp
-                terms.append(om * om / (swp * hp ** 3))
-        ref = math.fsum(terms + [-n0p])
-        assert abs(exact[i] - ref) <= 4e-16 * n0p, (i, exact[i], ref)
-        assert abs(f["vol_strain"][i] - exact[i]) <= 64 * 1.2e-16 * n0p
+            q = np.mod(x[j] - x[i] + 0.5 * w, w) - 0.5 * w        # the list's cut-off test (double)
+            if hp * hp - float(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) < 0:
+                continue
+            e2 = Fraction(0)
+            for k in range(3):
+                a = Fraction(x[j][k]) - Fraction(x[i][k])
+                if a > Fraction(w[k]) / 2:
+                    a -= Fraction(w[k])
+                elif a < -Fraction(w[k]) / 2:
+                    a += Fraction(w[k])
+                e2 += a * a
+            r = (D(e2.numerator) / D(e2.denominator)).sqrt()
+            om = 1 - r / D(hp)
+            acc += c * om * om
+        ref = float(acc - D(n0p))
+        assert abs(exact[i] - ref) <= 1e-17 * n0p, (i, exact[i], ref)
+        assert abs(f["vol_strain"][i] - exact[i]) <= 256 * 1.2e-16 * n0p
