@@ -5,6 +5,10 @@ type 0 below / type 1 above mid-height, in a 384x64x192 box with 3-layer walls o
 z faces, y periodic; 1,272,064 particles), FP64.  A step is one full iteration of the
 reference's loop body (source/main.cpp:386-431) for every particle, rebuilds included.
 
+`value` is timed over --steps steps from step --from-step (1000: the developed flow, where
+the lists are rebuilt every few steps as they are for most of a run); the first steps of
+the flow (from rest, lists rebuilt every 20-50 steps) are reported as `start_of_flow`.
+
 Prints ONE JSON line: steps/s (value), pair-interactions/s, the roofline of the dominant
 kernel from HIP events on the solver's stream, and the reference's own OpenMP build timed
 on a bounded sample of the same deck on this host.
@@ -79,18 +83,33 @@ def measured_traffic(workload: str, cluster_size: int, kernel: str):
     return float(k["bytes"]) if k else None
 
 
-def timed_run(s, args):
-    """warmup, timed steps (wall clock around the stream), then a HIP-event window"""
+def timed_steps(s, args, n):
+    """n steps timed by the wall clock around the stream (synchronised on both sides)"""
     stepper = s.step_graph if args.graph else s.step
-    stepper(args.warmup)
     s.sync()
     t0 = time.perf_counter()
-    stepper(args.steps)
+    stepper(n)
     s.sync()
-    elapsed = time.perf_counter() - t0
+    return n / (time.perf_counter() - t0)
+
+
+def timed_run(s, args):
+    """warmup, `start_of_flow` steps, advance to --from-step, the timed window (value), then a
+    HIP-event window of the same developed flow"""
+    stepper = s.step_graph if args.graph else s.step
+    stepper(args.warmup)
+    start = {"from_step": args.warmup, "steps": args.steps, "value": timed_steps(s, args, args.steps),
+             "unit": "steps/s"}
+    cur = s.stats()["steps"]
+    if cur < args.from_step:
+        stepper(args.from_step - cur)
+    rb0 = s.stats()["rebuilds"]
+    from_step = s.stats()["steps"]
+    sps = timed_steps(s, args, args.steps)
+    rebuilds = s.stats()["rebuilds"] - rb0
     k2 = max(10, args.steps // 4)
     tm = s.step_timed(k2)
-    return args.steps / elapsed, tm, k2, args.warmup + args.steps + k2
+    return sps, tm, k2, {"start_of_flow": start, "from_step": from_step, "rebuilds": rebuilds}
 
 
 def field_errors(a, b):
@@ -136,10 +155,11 @@ def batch_leg(args, Solver):
 SUR_SCALES = dict(u_scale=0.1, a_scale=10.0, dens_scale=50.0)
 
 
-def surrogate_leg(args, deck, plain_sps):
+def surrogate_leg(args, deck):
     """Learned 2 dt steps on the same deck: train on the deck's own plain trajectory (from a
-    developed flow), then time bare surrogate steps and judged blocks (K surrogate steps,
-    one-step judge against two plain steps, rollback above the bound)."""
+    developed flow), then time plain steps, bare surrogate steps and judged blocks (K
+    surrogate steps, one-step judge against two plain steps, rollback above the bound) in the
+    same stretch of the flow."""
     from mphexplicit_amd.solver import Solver, Surrogate
     s = Solver.from_deck(deck)
     s.step(args.sur_warm)
@@ -148,6 +168,7 @@ def surrogate_leg(args, deck, plain_sps):
     t0 = time.perf_counter()
     losses = sur.train(args.sur_samples, args.sur_iters)
     t_train = time.perf_counter() - t0
+    plain_sps = timed_steps(s, args, 2 * args.sur_steps)
     sur.step(2)
     s.sync()
     t0 = time.perf_counter()
@@ -180,7 +201,9 @@ def main():
     ap.add_argument("--workload", default="dambreak3d_1m")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--graph", action="store_true", help="replay a captured HIP graph per step")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--from-step", type=int, default=1000,
+                    help="step from which the timed window starts (the developed flow)")
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-case batch leg")
@@ -193,9 +216,8 @@ def main():
     ap.add_argument("--sur-blocks", type=int, default=8)
     ap.add_argument("--sur-k", type=int, default=10)
     ap.add_argument("--sur-tol", type=float, default=0.01)
-    ap.add_argument("--late-from", type=int, default=1000,
-                    help="step from which the developed-flow window is timed (lists rebuilt often)")
-    ap.add_argument("--late-steps", type=int, default=200, help="steps of the developed-flow window; 0: skip")
+    ap.add_argument("--late-steps", type=int, default=200,
+                    help="steps of a longer developed-flow window after the timed one; 0: skip")
     ap.add_argument("--error-steps", type=int, default=500,
                     help="steps after which the mixed leg's fields are compared with FP64")
     args = ap.parse_args()
@@ -212,7 +234,7 @@ def main():
     t0 = time.perf_counter()
     s = Solver.from_deck(deck, prec)
     log("create %.2fs" % (time.perf_counter() - t0))
-    steps_per_s, tm, k2, done = timed_run(s, args)
+    steps_per_s, tm, k2, win = timed_run(s, args)
     st = s.stats()
     surface = bool(any(t > 0 for t in deck.data.surface_tension))
     fa, fb = FLOPS_PER_PAIR[surface]
@@ -222,7 +244,7 @@ def main():
     dom = max(("sweep_a", "sweep_b"), key=lambda k: phases[k])
     fl = (fa if dom == "sweep_a" else fb) * pairs_per_step
     achieved = fl / (phases[dom] * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+    roofline = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS,
                 "traffic": measured_traffic(deck.name, st["cluster_size"], "k_" + dom) if prec == MPH_FP64 else None,
                 "kernel": "k_" + dom,
@@ -236,11 +258,16 @@ def main():
         # the same deck with FP32 pair math: timing, and the error of every field against the
         # FP64 run after the same number of steps
         m = Solver.from_deck(deck, MPH_MIXED)
-        m_sps, m_tm, _, m_done = timed_run(m, args)
-        n_err = max(args.error_steps, done)
-        s.step(n_err - done)
-        m.step(n_err - m_done)
-        errs = field_errors(m.fields(names=ERROR_FIELDS), s.fields(names=ERROR_FIELDS))
+        m_sps, m_tm, _, m_win = timed_run(m, args)
+        # the error after the same number of steps from the same state: a fresh FP64 and mixed
+        # pair run --error-steps steps
+        e64, e32 = Solver.from_deck(deck, MPH_FP64), Solver.from_deck(deck, MPH_MIXED)
+        e64.step(args.error_steps)
+        e32.step(args.error_steps)
+        n_err = args.error_steps
+        errs = field_errors(e32.fields(names=ERROR_FIELDS), e64.fields(names=ERROR_FIELDS))
+        e64.close()
+        e32.close()
         m_ph = {"sweep_a": m_tm["sweep_a_ms"] / k2, "sweep_b": m_tm["sweep_b_ms"] / k2}
         m_dom = max(m_ph, key=lambda k: m_ph[k])
         m_ach = (fa if m_dom == "sweep_a" else fb) * (m_tm["pairs"] / k2) / (m_ph[m_dom] * 1e-3) / 1e12
@@ -253,23 +280,15 @@ def main():
         m.close()
     late = None
     if args.late_steps > 0:
-        # the same FP64 run later in the flow, where the faster fluid rebuilds the lists every
-        # few steps: the headline window above sees one or two rebuilds
+        # a longer window of the same developed flow (the timed window is --steps long)
         cur = s.stats()["steps"]
-        if cur < args.late_from:
-            s.step(args.late_from - cur)
-        s.sync()
         rb0 = s.stats()["rebuilds"]
-        t0 = time.perf_counter()
-        s.step(args.late_steps)
-        s.sync()
-        el = time.perf_counter() - t0
-        late = {"from_step": max(cur, args.late_from), "steps": args.late_steps, "value": args.late_steps / el,
+        late = {"from_step": cur, "steps": args.late_steps, "value": timed_steps(s, args, args.late_steps),
                 "unit": "steps/s", "rebuilds": s.stats()["rebuilds"] - rb0}
     batch = None if args.no_batch else batch_leg(args, Solver)
     surrogate = None
     if prec == MPH_FP64 and not args.no_surrogate:
-        surrogate = surrogate_leg(args, deck, steps_per_s)
+        surrogate = surrogate_leg(args, deck)
     cpu = None
     if not args.no_cpu_baseline:
         cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
@@ -294,8 +313,12 @@ def main():
                    "precision": args.precision, "dt": deck.data.dt, "graph": bool(args.graph),
                    "list_width": st["list_width"],
                    "cluster_size": st["cluster_size"]},
+        "timed_window": {"from_step": win["from_step"], "steps": args.steps, "rebuilds": win["rebuilds"],
+                         "note": "value: steps/s of the --steps steps from step --from-step (developed flow)"},
+        "start_of_flow": win["start_of_flow"],
         "pair_interactions_per_s": pairs_per_step * steps_per_s,
         "pairs_per_step": pairs_per_step,
+        "slots_per_pair": st["eval_slots"] / pairs_per_step if pairs_per_step else None,
         "simulated_seconds_per_s": deck.data.dt * steps_per_s,
         "rebuilds_in_profiled_window": tm["rebuilds"],
         "phase_ms": phases,
@@ -307,7 +330,10 @@ def main():
         "surrogate": surrogate,
     }
     if cpu:
+        # the CPU sample is the start of the flow (steps 2-21, few rebuilds); the GPU's
+        # start-of-flow window is the like-for-like comparison
         out["speedup_vs_cpu"] = steps_per_s / cpu["value"]
+        out["speedup_vs_cpu_same_window"] = win["start_of_flow"]["value"] / cpu["value"]
     print(json.dumps(out), flush=True)
     s.close()
 

@@ -98,6 +98,9 @@ typedef struct mph_stats {
     int max_list_len;                 /* longest cluster list since the lists were sized */
     int cluster_size;                 /* particles per cluster tile, picked by the autotuner */
     int clusters;                     /* clusters at the last rebuild */
+    long long eval_slots;             /* (member, list entry) evaluations of one pair sweep over
+                                         the current lists, padding and non-interacting entries
+                                         included */
 } mph_stats;
 
 /* Per-phase device time of mph_step_timed, summed over its steps. */

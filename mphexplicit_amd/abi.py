@@ -65,6 +65,7 @@ class MphStats(C.Structure):
         ("max_list_len", C.c_int),
         ("cluster_size", C.c_int),
         ("clusters", C.c_int),
+        ("eval_slots", C.c_longlong),
     ]
 
 

@@ -1308,8 +1308,16 @@ extern "C" int mph_get_stats(const mph_ctx* cc, mph_stats* out)
     out->rebuilds = 0;
     for (long long r : rb) out->rebuilds = std::max(out->rebuilds, r);
     out->list_entries = 0;
+    out->eval_slots = 0;
+    // a sweep's wavefront walks a list in blocks of 64 entries, K steps of 64/K per full
+    // block and ceil(rest / (64/K)) for the last, each step 64 lane evaluations
+    const int S = 64 / c->K;
     for (int k = 0; k < c->ncases; ++k)
-        for (int g = 0; g < cnt[k]; ++g) out->list_entries += len[c->cases[k].k.cl_base + g];
+        for (int g = 0; g < cnt[k]; ++g) {
+            const int l = len[c->cases[k].k.cl_base + g];
+            out->list_entries += l;
+            out->eval_slots += 64ll * ((long long)c->K * (l / 64) + (l % 64 + S - 1) / S);
+        }
     out->list_width = c->width;
     out->max_list_len = mx;
     out->cluster_size = c->K;

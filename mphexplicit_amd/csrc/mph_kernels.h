@@ -103,14 +103,24 @@ __device__ __forceinline__ double image_near(double x, double xf, double hw, dou
 // the reference's Mod form only by the rounding of the +-W/2 round trip.  Discrete
 // decisions (list cut-off, cell keys) keep the reference's form (rel3).
 
-// 1/sqrt(q2) for the pair sweeps: the hardware estimate y (measured on MI355X: at most
-// 5.2e-8 relative error) and one third-order correction.  With e = 1 - q2 y^2,
-//   1/sqrt(q2) = y (1 - e)^(-1/2) = y (1 + e/2 + 3e^2/8 + O(e^3)),
-// so the truncation error is below 0.3 |e|^3 ~ 4e-23 and the result is within a few ulp
-// (rounding of e, of the polynomial and of the final fma): one instruction more than a
-// Newton step, whose 1.5 e^2 ~ 4e-15 (37 ulp) error the pressure of the 1M-particle dam
-// break amplified to 1.9e-12 of the field (P is a difference of sums near N0p).
+// 1/sqrt(q2) for the pair sweeps, from the hardware estimate y (measured on MI355X: at most
+// 5.2e-8 relative error).  With e = 1 - q2 y^2,
+//   1/sqrt(q2) = y (1 - e)^(-1/2) = y (1 + e/2 + 3e^2/8 + O(e^3)).
+// rsqrt_pair (sweep B's forces): one Newton step, y (1 + e/2), at most 1.5 e^2 ~ 4.1e-15
+// relative (37 ulp).  rsqrt_pair3 (sweep A's densities): the third-order correction, below
+// 0.3 |e|^3 ~ 4e-23 before rounding, within 1 ulp (one instruction more): VolStrainP =
+// sum w_P - N0p is a difference of sums near N0p, and the pressure it sets is sensitive to
+// every rounding of r.  On the 1M-particle dam break the GPU's VolStrainP is within 1.1e-15
+// of the exact sum (the reference's own FP64 sum: 7.6e-15).  In sweep B the third-order step
+// measured 6% slower and the forces do not need it.
 __device__ __forceinline__ double rsqrt_pair(double q2)
+{
+    const double y = __builtin_amdgcn_rsq(q2);
+    const double e = fma(-(0.5 * q2) * y, y, 0.5);
+    return fma(y, e, y);
+}
+
+__device__ __forceinline__ double rsqrt_pair3(double q2)
 {
     const double y = __builtin_amdgcn_rsq(q2);
     const double e = fma(-(q2 * y), y, 1.0);
@@ -1132,7 +1142,7 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         // elsewhere (x_ij = 0, u.x_ij = 0)
         const double q2 = fmax(fma(dz, dz, fma(dy, dy, dx * dx)), 1.0e-300);
         if (COUNT) acc.np += q2 <= C.hmax2;
-        const double ir = rsqrt_pair(q2);
+        const double ir = rsqrt_pair3(q2);
         const double r = q2 * ir;
         const double oma = fmax(fma(-r, C.inv_ha, 1.0), 0.0);
         const double ratio = (R1 || C.ratio_one) ? 1.0 : tab[16 + ti * 4 + (tag_of(q.p.w) & 3)];
