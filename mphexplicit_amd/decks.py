@@ -248,18 +248,99 @@ def dam_break_1m(**kw) -> Deck:
 
 
 # --------------------------------------------------------------------------------------
-# the reference's two sample decks (results/*/box.boid, box.data), expanded like
-# generator/generator.cpp:170-193 and written with its %e precision (:197-217)
+# .boid -> .grid, as the reference's generator does it (generator/generator.cpp)
 
 
-def _boid_expand(cuboids):
+@dataclass
+class Boid:
+    """Contents of a ``.boid`` file (generator/generator.cpp:49-165): the domain, the particle
+    distance written into the .grid header, and cuboids (spacing, type, lower, upper,
+    velocity) filled with particles."""
+    particle_distance: float
+    lower: tuple
+    upper: tuple
+    cuboids: list
+
+
+def parse_boid(text: str) -> Boid:
+    """Parse ``.boid`` text the way generator/generator.cpp:49-165 reads it: top-level lines
+    whose first word is ParticleDistance, LowerDomain, UpperDomain or StartCuboid ('#' lines
+    skipped, other lines ignored); after StartCuboid, whitespace-separated tokens (across
+    lines) Spacing, Type, Lower, Upper, Velocity until EndCuboid, whose line's remainder is
+    read as the next line.  Raises ValueError where the generator reports an error."""
+    pd, lo, hi, cubs = -1.0, [0.0, 0.0, 0.0], [0.0, 0.0, 0.0], []
+    lines = text.splitlines(keepends=True)
+    k = 0
+    pending = None                     # remainder of the line that ended a cuboid
+    while pending is not None or k < len(lines):
+        if pending is not None:
+            buf, pending = pending, None
+        else:
+            buf, k = lines[k], k + 1
+        if buf.startswith("#"):
+            continue
+        w = buf.split()
+        if not w:
+            continue
+        if w[0] == "ParticleDistance":
+            if len(w) < 2:
+                raise ValueError("ParticleDistance count not 1")
+            pd = float(w[1])
+        elif w[0] in ("LowerDomain", "UpperDomain"):
+            if len(w) < 4:
+                raise ValueError("%s count not 3" % w[0])
+            (lo if w[0] == "LowerDomain" else hi)[:] = [float(a) for a in w[1:4]]
+        elif w[0] == "StartCuboid":
+            # token stream from the next line on (fscanf after fgets)
+            rest = "".join(lines[k:])
+            toks = rest.split()
+            cub = {"Spacing": None, "Type": None, "Lower": None, "Upper": None, "Velocity": None}
+            t = 0
+            while True:
+                if t >= len(toks):
+                    raise ValueError("cuboid without EndCuboid")
+                tok = toks[t]
+                t += 1
+                if tok == "EndCuboid":
+                    break
+                if tok == "Spacing":
+                    cub["Spacing"] = float(toks[t]); t += 1
+                elif tok == "Type":
+                    cub["Type"] = int(toks[t]); t += 1
+                elif tok in ("Lower", "Upper", "Velocity"):
+                    cub[tok] = tuple(float(a) for a in toks[t:t + 3]); t += 3
+                else:
+                    raise ValueError("no such indication: %s" % tok)
+            missing = [key for key, v in cub.items() if v is None]
+            if missing:
+                raise ValueError("cuboid without %s" % ", ".join(missing))
+            cubs.append((cub["Spacing"], cub["Type"], cub["Lower"], cub["Upper"], cub["Velocity"]))
+            # continue with the rest of the line that holds EndCuboid
+            consumed, line_k, pos = 0, k, 0
+            for line_k in range(k, len(lines)):
+                n_here = len(lines[line_k].split())
+                if consumed + n_here >= t:
+                    break
+                consumed += n_here
+            words = lines[line_k].split()
+            after = words[t - consumed:]
+            k = line_k + 1
+            if after:
+                pending = " ".join(after) + "\n"
+    return Boid(pd, tuple(lo), tuple(hi), cubs)
+
+
+def _boid_particles(cuboids):
+    """types, positions and velocities of the cuboids' particles (generator.cpp:170-193):
+    per axis count = round(width / spacing) and spacing = width / count, centres from
+    lower + spacing/2 while below upper - 0.49 spacing, accumulated by repeated addition"""
     types, pos, vel = [], [], []
     for space, ctype, lower, upper, velocity in cuboids:
         lower = np.array(lower, float)
         upper = np.array(upper, float)
         width = upper - lower
-        # C round() rounds halves away from zero; widths here are positive
-        count = [int(math.floor(width[d] / space + 0.5)) for d in range(3)]
+        # C round() rounds halves away from zero
+        count = [int(math.copysign(math.floor(abs(width[d] / space) + 0.5), width[d] / space)) for d in range(3)]
         spacing = [width[d] / count[d] for d in range(3)]
         px = lower[0] + 0.5 * spacing[0]
         while px < upper[0] - 0.49 * spacing[0]:
@@ -273,6 +354,38 @@ def _boid_expand(cuboids):
                     pz += spacing[2]
                 py += spacing[1]
             px += spacing[0]
+    return types, pos, vel
+
+
+def grid_text_from_boid(boid: Boid) -> str:
+    """The .grid file the reference's generator writes for a .boid (generator.cpp:197-217)"""
+    types, pos, vel = _boid_particles(boid.cuboids)
+    lo, hi = boid.lower, boid.upper
+    out = ["%f\n" % 0.0,
+           "%d %e  %e %e %e  %e %e %e\n" % (len(types), boid.particle_distance, lo[0], hi[0], lo[1], hi[1], lo[2],
+                                           hi[2])]
+    out += ["%d   %e %e %e    %e %e %e\n" % (t, p[0], p[1], p[2], v[0], v[1], v[2])
+            for t, p, v in zip(types, pos, vel)]
+    return "".join(out)
+
+
+def boid_to_grid(case: str) -> str:
+    """``GeneratorForMph <case>`` (generator.cpp:37-47): read <case>.boid, write <case>.grid"""
+    with open(case + ".boid") as f:
+        boid = parse_boid(f.read())
+    path = case + ".grid"
+    with open(path, "w") as f:
+        f.write(grid_text_from_boid(boid))
+    return path
+
+
+# --------------------------------------------------------------------------------------
+# the reference's two sample decks (results/*/box.boid, box.data), expanded like
+# generator/generator.cpp:170-193 and written with its %e precision (:197-217)
+
+
+def _boid_expand(cuboids):
+    types, pos, vel = _boid_particles(cuboids)
     t = np.array(types, np.int32)
     # the generator prints %e (7 significant digits); the solver reads those decimals back
     x = np.array([[float("%e" % c) for c in p] for p in pos])
@@ -280,14 +393,22 @@ def _boid_expand(cuboids):
     return t, x, v
 
 
+# the cuboids of the two sample .boid files (results/*/box.boid), as data
+SAMPLE_BRIDGE_CUBOIDS = [
+    (0.0001, 1, (-0.002, 0.000, -0.00005), (0.002, 0.003, 0.00005), (0.0, 0.0, 0.0)),
+    (0.0001, 2, (-0.005, -0.001, -0.00005), (0.005, 0.000, 0.00005), (0.0, 0.0, 0.0)),
+    (0.0001, 2, (-0.005, 0.003, -0.00005), (0.005, 0.004, 0.00005), (0.0, 0.0, 0.0)),
+]
+SAMPLE_SQUARE_DROP_CUBOIDS = [
+    (0.04, 1, (0.2, 0.68, -0.02), (1.2, 1.68, 0.02), (0.0, -0.8, 0.0)),
+    (0.04, 2, (0.0, 0.0, -0.02), (0.4, 0.6, 0.02), (0.0, 0.0, 0.0)),
+    (0.04, 2, (0.0, -0.20, -0.02), (3.0, 0.00, 0.02), (0.0, 0.0, 0.0)),
+]
+
+
 def sample_bridge() -> Deck:
     """results/Bridge_l00100_deg000_re35_mu01_g98 (2D liquid bridge between plates)."""
-    cub = [
-        (0.0001, 1, (-0.002, 0.000, -0.00005), (0.002, 0.003, 0.00005), (0, 0, 0)),
-        (0.0001, 2, (-0.005, -0.001, -0.00005), (0.005, 0.000, 0.00005), (0, 0, 0)),
-        (0.0001, 2, (-0.005, 0.003, -0.00005), (0.005, 0.004, 0.00005), (0, 0, 0)),
-    ]
-    t, x, v = _boid_expand(cub)
+    t, x, v = _boid_expand(SAMPLE_BRIDGE_CUBOIDS)
     data = DataParams(dt=6.25e-6, output_interval=1.0, vtk_output_interval=2.0e-3, end_time=2.0e-2,
                       radius_ratio_a=3.5, radius_ratio_p=1.75, radius_ratio_v=3.5,
                       density=(1000.0,) * 4, bulk_modulus=(1.0e4,) * 4, bulk_viscosity=(0.1,) * 4,
@@ -300,12 +421,7 @@ def sample_bridge() -> Deck:
 
 def sample_square_drop() -> Deck:
     """results/SquareDropSkip_l00400_rv35_rp17 (2D square drop falling onto a step)."""
-    cub = [
-        (0.04, 1, (0.2, 0.68, -0.02), (1.2, 1.68, 0.02), (0.0, -0.8, 0.0)),
-        (0.04, 2, (0.0, 0.0, -0.02), (0.4, 0.6, 0.02), (0, 0, 0)),
-        (0.04, 2, (0.0, -0.20, -0.02), (3.0, 0.00, 0.02), (0, 0, 0)),
-    ]
-    t, x, v = _boid_expand(cub)
+    t, x, v = _boid_expand(SAMPLE_SQUARE_DROP_CUBOIDS)
     data = DataParams(dt=5.0e-4, output_interval=1.0, vtk_output_interval=5.0e-2, end_time=1.0,
                       radius_ratio_a=3.5, radius_ratio_p=1.75, radius_ratio_v=3.5,
                       density=(1.0e3,) * 4, bulk_modulus=(1.0e6,) * 4, bulk_viscosity=(2.0e3,) * 4,
@@ -493,3 +609,11 @@ def batch_cases(count=64, seed=4, n_min=2.0e3, n_max=6.0e4, l0=1.0e-3) -> List[D
         decks.append(Deck("batch%02d_%dd_%s" % (k, dim, "bridge" if bridge else "drop"), dim, data,
                           l0, lo, hi, t, x, np.zeros((len(x), 3)), steps=1000))
     return decks
+
+
+if __name__ == "__main__":
+    # python -m mphexplicit_amd.decks <case>: the reference's `GeneratorForMph <case>`
+    import sys
+    if len(sys.argv) != 2:
+        sys.exit("usage: python -m mphexplicit_amd.decks <case>   (reads <case>.boid, writes <case>.grid)")
+    print(boid_to_grid(sys.argv[1]))

@@ -2,8 +2,10 @@
 
 Runs the reference build (oracle/_ref, compiled from source/main.cpp by oracle/Makefile) on
 a small 2D droplet deck with .prof output every 5 steps and .vtk output every 10 steps over
-20 steps, in a temporary directory, and stores the names, times and particle rows of its
-.prof files and the names of its .vtk files in driver_droplet2d.npz (data only).
+20 steps, in a temporary directory, and stores in driver_droplet2d.npz (data only): the
+names, times and particle rows of its .prof files, and the names and every array of its
+.vtk files (points, label, DensityA, VolStrainP, VirialPressureAtParticle, neighbor,
+velocity, force: the layout of main.cpp:698-834).
 
 Run from the repository root in the container that has the reference:
     python tests/golden/make_driver_golden.py
@@ -39,6 +41,37 @@ def read_prof(path):
     return t, rows
 
 
+VTK_SCALARS = ("label", "DensityA", "VolStrainP", "VirialPressureAtParticle", "neighbor")
+VTK_VECTORS = ("velocity", "force")
+
+
+def read_vtk(path):
+    """Arrays of a legacy ASCII .vtk file in the layout of main.cpp:698-834:
+    {"points": [n,3], "<scalar>": [n], "<vector>": [n,3]}"""
+    with open(path) as f:
+        tok = f.read().split("\n")
+    out, i = {}, 0
+    while i < len(tok):
+        line = tok[i].split()
+        if line[:1] == ["POINTS"]:
+            n = int(line[1])
+            out["points"] = np.array([[float(a) for a in tok[i + 1 + k].split()] for k in range(n)])
+            i += 1 + n
+            continue
+        if line[:1] == ["SCALARS"]:
+            n = len(out["points"])
+            out[line[1]] = np.array([float(tok[i + 2 + k]) for k in range(n)])
+            i += 2 + n
+            continue
+        if line[:1] == ["VECTORS"]:
+            n = len(out["points"])
+            out[line[1]] = np.array([[float(a) for a in tok[i + 1 + k].split()] for k in range(n)])
+            i += 1 + n
+            continue
+        i += 1
+    return out
+
+
 def main():
     lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libmphref2d.so"))
     d = driver_deck()
@@ -64,6 +97,9 @@ def main():
             times.append(t)
             out["prof_%d" % k] = rows
         out["prof_times"] = np.array(times)
+        for k, name in enumerate(vtks):
+            for field, arr in read_vtk(os.path.join(tmp, name)).items():
+                out["vtk_%d_%s" % (k, field)] = arr
     np.savez_compressed(os.path.join(ROOT, "tests", "golden", "driver_droplet2d.npz"), **out)
     print("wrote", len(out["prof_names"]), "prof and", len(out["vtk_names"]), "vtk names")
 
