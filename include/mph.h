@@ -195,6 +195,19 @@ int mph_compute_virial(mph_ctx *ctx);
  * test at the rebuild positions.  Call with ind == NULL to get only ptr (ptr[n] = total). */
 int mph_get_neighbors(mph_ctx *ctx, int case_id, long long *ptr, int *ind);
 
+/* mph_get_neighbors into device memory, allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture elsewhere in the
+ * process, serving loops, harnesses that forbid hipMalloc).  ptr: device array of n+1
+ * (n = mph_particle_count(ctx, case_id)); ind: device array of ind_capacity entries, or NULL
+ * for the row starts only; *total (may be NULL) receives ptr[n].  Same rows as
+ * mph_get_neighbors (original ids, each row ascending, self included).  The library's own
+ * scratch is a 64 KiB array per GPU: cases of up to 8160 * 4096 (33.4M) particles; larger
+ * ones are rejected with MPH_ERR_ARG, and so is an ind too small for the lists (the needed
+ * size is reported in *total).  Calls on one GPU share that scratch: calls that overlap
+ * (from several threads or streams) race.  Synchronises the context's stream. */
+int mph_get_neighbors_noalloc(mph_ctx *ctx, int case_id, long long *ptr, int *ind, long long ind_capacity,
+                              long long *total);
+
 int mph_get_stats(const mph_ctx *ctx, mph_stats *out);
 
 void mph_destroy(mph_ctx *ctx);

@@ -17,6 +17,7 @@
 #include <string>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "mph.h"
@@ -863,6 +864,8 @@ static int choose_cluster_size(mph_ctx* c, int* K)
     return MPH_OK;
 }
 
+static int resolve_scratch(int device);
+
 static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* counts, const int* type,
                  const double* x, const double* v)
 {
@@ -870,6 +873,8 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count <= 0)
         return fail(MPH_ERR_DEVICE, "no GPU available");
     HIP_TRY(hipGetDevice(&c->device));
+    int rs = resolve_scratch(c->device);
+    if (rs) return rs;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&c->ev_flag, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&c->ev_cap, hipEventDisableTiming));
@@ -1289,6 +1294,98 @@ extern "C" int mph_get_neighbors(mph_ctx* c, int case_id, long long* ptr, int* i
             std::copy(row.begin(), row.end(), ind + ptr[o]);
         }
     }
+    return MPH_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Allocation-free entry point: mph_get_neighbors on the device, into the caller's device
+// buffers.  Its scratch (the scan's tile sums and an error counter) is this file-scope
+// array; every GPU holds its own copy until the process ends, and the host addresses it only
+// through the pointer hipGetSymbolAddress gives for each device (resolved once per device,
+// at the first mph_create on it).  Calls that overlap on one GPU share it and race.
+constexpr int kScratchBytes = 64 * 1024;
+constexpr int kScratchHead = 256;                // error counter, then the tile sums
+constexpr int kScratchTiles = (kScratchBytes - kScratchHead) / (int)sizeof(long long);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static int resolve_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(MPH_ERR_DEVICE, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return MPH_OK;
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    return MPH_OK;
+}
+
+static bool is_device_pointer(const void* p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();            // an unregistered host pointer leaves an error behind
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+extern "C" int mph_get_neighbors_noalloc(mph_ctx* c, int case_id, long long* ptr, int* ind, long long ind_capacity,
+                                         long long* total)
+{
+    if (!c || !ptr || case_id < 0 || case_id >= c->ncases)
+        return fail(MPH_ERR_ARG, "mph_get_neighbors_noalloc: bad argument");
+    if (!is_device_pointer(ptr) || (ind && !is_device_pointer(ind)))
+        return fail(MPH_ERR_ARG, "mph_get_neighbors_noalloc: ptr and ind must be device memory");
+    unsigned char* sp = g_scratch_ptr[c->device];
+    if (!sp) return fail(MPH_ERR_STATE, "mph_get_neighbors_noalloc: scratch not resolved for device %d", c->device);
+    const CaseConst& k = c->cases[case_id].k;
+    const int n = k.end - k.begin;
+    const int tiles = (n + kCsrTile - 1) / kCsrTile;
+    if (tiles > kScratchTiles)
+        return fail(MPH_ERR_ARG, "mph_get_neighbors_noalloc: a case of %d particles exceeds the scratch limit of %d",
+                    n, kScratchTiles * kCsrTile);
+    hipStream_t s = c->stream;
+    int* err = reinterpret_cast<int*>(sp);
+    long long* sums = reinterpret_cast<long long*>(sp + kScratchHead);
+    hipLaunchKernelGGL(k_scratch_clear, dim3(1), dim3(256), 0, s, err, 1);
+    hipLaunchKernelGGL(k_csr_counts, dim3((n + 255) / 256), dim3(256), 0, s, k.begin, n, c->pidx.p, c->nout.p, ptr);
+    hipLaunchKernelGGL(k_csr_tile_sums, dim3(tiles), dim3(kCsrThreads), 0, s, ptr + 1, n, sums);
+    hipLaunchKernelGGL(k_csr_scan_sums, dim3(1), dim3(1024), 0, s, sums, tiles);
+    hipLaunchKernelGGL(k_csr_tile_scan, dim3(tiles), dim3(kCsrThreads), 0, s, ptr + 1, n, sums);
+    int rc = launch_ok("neighbour rows: counts");
+    if (rc) return rc;
+    long long tot = 0;
+    HIP_TRY(hipMemcpyAsync(&tot, ptr + n, sizeof(long long), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (total) *total = tot;
+    if (!ind) return MPH_OK;
+    if (tot > ind_capacity)
+        return fail(MPH_ERR_ARG, "mph_get_neighbors_noalloc: ind holds %lld entries, the lists need %lld",
+                    ind_capacity, tot);
+    int pad = 1;
+    while (pad < c->width + 1) pad <<= 1;
+    const int waves = std::max(1, std::min(4, (int)((64 * 1024) / ((size_t)pad * sizeof(int)))));
+    int ncl = 0;
+    HIP_TRY(hipMemcpyAsync(c->h_scalar, c->cl_count.p + case_id, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    ncl = *c->h_scalar;
+    if (ncl > 0) {
+        const int grid = std::max(1, std::min((ncl + waves - 1) / waves, 20480));
+        hipLaunchKernelGGL(k_csr_rows, dim3(grid), dim3(64 * waves), (size_t)pad * sizeof(int) * waves, s,
+                           c->d_cases.p, case_id, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width,
+                           c->snap.p, c->pidx.p, c->nout.p, ptr, ind, pad, err);
+        if ((rc = launch_ok("neighbour rows"))) return rc;
+    }
+    int bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (bad) return fail(MPH_ERR_STATE, "mph_get_neighbors_noalloc: %d rows differ from their NeighborCount", bad);
     return MPH_OK;
 }
 

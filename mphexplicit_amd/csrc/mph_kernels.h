@@ -1506,3 +1506,189 @@ k_virial(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
 }
 
 }  // namespace mph
+
+namespace mph {
+
+// -------------------------------------------------------------------------------------
+// Neighbour lists in CSR form on the device, keyed by original particle id: the reference's
+// NeighborPtr / NeighborInd (main.cpp:1329-1596) mapped through ParticleIndex, each row
+// ascending with self included (mph_get_neighbors_noalloc).  Row lengths are the lists'
+// NeighborCount; a scan gives the row starts; each member of each cluster then rebuilds its
+// row from the cluster's union list with the reference's exact cut-off test at the rebuild
+// positions, and sorts it in LDS.  The scan's tile sums and an error counter live in a
+// caller-free scratch buffer.
+
+constexpr int kCsrThreads = 256;
+constexpr int kCsrItems = 16;
+constexpr int kCsrTile = kCsrThreads * kCsrItems;
+
+__global__ void __launch_bounds__(256)
+k_scratch_clear(int* __restrict__ w, int count)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) w[k] = 0;
+}
+
+// ptr[o + 1] = NeighborCount of the particle with original id o, ptr[0] = 0
+__global__ void __launch_bounds__(256)
+k_csr_counts(int begin, int n, const int* __restrict__ pidx, const int* __restrict__ nout,
+             long long* __restrict__ ptr)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) ptr[0] = 0;
+    if (k < n) ptr[pidx[begin + k] + 1] = nout[begin + k];
+}
+
+__global__ void __launch_bounds__(kCsrThreads)
+k_csr_tile_sums(const long long* __restrict__ v, int n, long long* __restrict__ sums)
+{
+    const int base = blockIdx.x * kCsrTile;
+    long long s = 0;
+    for (int k = threadIdx.x; k < kCsrTile; k += kCsrThreads) s += base + k < n ? v[base + k] : 0ll;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    __shared__ long long ws[kCsrThreads / 64];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int w = 0; w < kCsrThreads / 64; ++w) t += ws[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+// exclusive scan of `total` tile sums in place, one workgroup of 1024 threads
+__global__ void __launch_bounds__(1024)
+k_csr_scan_sums(long long* __restrict__ data, int total)
+{
+    __shared__ long long part[1024];
+    const int tid = threadIdx.x;
+    const int chunk = (total + 1023) / 1024;
+    const int lo = min(total, tid * chunk), hi = min(total, lo + chunk);
+    long long s = 0;
+    for (int k = lo; k < hi; ++k) s += data[k];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const long long v = tid >= off ? part[tid - off] : 0ll;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    long long run = part[tid] - s;
+    for (int k = lo; k < hi; ++k) {
+        const long long v = data[k];
+        data[k] = run;
+        run += v;
+    }
+}
+
+// inclusive scan of each tile in place, plus the tile's offset
+__global__ void __launch_bounds__(kCsrThreads)
+k_csr_tile_scan(long long* __restrict__ v, int n, const long long* __restrict__ offsets)
+{
+    __shared__ long long part[kCsrThreads];
+    const int base = blockIdx.x * kCsrTile + threadIdx.x * kCsrItems;
+    long long x[kCsrItems], s = 0;
+#pragma unroll
+    for (int q = 0; q < kCsrItems; ++q) {
+        x[q] = base + q < n ? v[base + q] : 0ll;
+        s += x[q];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < kCsrThreads; off <<= 1) {
+        const long long t = (int)threadIdx.x >= off ? part[threadIdx.x - off] : 0ll;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    long long run = offsets[blockIdx.x] + part[threadIdx.x] - s;
+#pragma unroll
+    for (int q = 0; q < kCsrItems; ++q) {
+        run += x[q];
+        if (base + q < n) v[base + q] = run;
+    }
+}
+
+// one wavefront per cluster of case `cc`, member by member: the row of a member with a list
+// is itself plus every union entry within the cut-off at the rebuild positions (the
+// reference's test, main.cpp:1398-1408), as original ids, sorted ascending in LDS (bitonic,
+// rows padded to `pad`, a power of two above the list width); a row whose length differs
+// from NeighborCount raises err
+__global__ void __launch_bounds__(256)
+k_csr_rows(const CaseConst* __restrict__ cases, int cc, const int4* __restrict__ cl_info,
+           const int* __restrict__ cl_count, const int* __restrict__ cl_nbr, const int* __restrict__ cl_len,
+           int width, const double4* __restrict__ snap, const int* __restrict__ pidx,
+           const int* __restrict__ nout, const long long* __restrict__ ptr, int* __restrict__ ind, int pad,
+           int* __restrict__ err)
+{
+    extern __shared__ int rows[];
+    const CaseConst& C = cases[cc];
+    const int nwv = (int)(blockDim.x >> 6);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int* row = rows + (size_t)wv * pad;
+    const int ncl = cl_count[cc];
+    for (int q = blockIdx.x * nwv + wv; q < ncl; q += gridDim.x * nwv) {
+        const int g = C.cl_base + q;
+        const int4 info = cl_info[g];
+        const int len = cl_len[g];
+        const int* list = cl_nbr + (size_t)g * width;
+        for (int m = 0; m < info.y; ++m) {
+            const int i = info.x + m;
+            const int want = nout[i];
+            if (want == 0) continue;                       // no list (main.cpp:1416)
+            const double4 pi = snap[i];
+            int cnt = 0;
+            for (int base = 0; base < len; base += 64) {
+                const int e = base + lane;
+                bool in = false;
+                int o = 0;
+                if (e < len) {
+                    const int j = list[e];
+                    const double4 pj = snap[j];
+                    const double dx = rel_exact(pj.x, pi.x, C.hwid[0], C.dwid[0]);
+                    const double dy = rel_exact(pj.y, pi.y, C.hwid[1], C.dwid[1]);
+                    const double dz = rel_exact(pj.z, pi.z, C.hwid[2], C.dwid[2]);
+                    in = j != i && dx * dx + dy * dy + dz * dz <= C.cut2;
+                    o = pidx[j];
+                }
+                const unsigned long long b = __ballot(in);
+                const int at = cnt + (int)__popcll(b & below);
+                if (in && at < pad - 1) row[at] = o;
+                cnt += (int)__popcll(b);
+            }
+            if (cnt + 1 != want) {                           // cannot happen with consistent lists
+                if (lane == 0) atomicAdd(err, 1);
+                continue;
+            }
+            if (lane == 0) row[cnt] = pidx[i];               // self
+            const int L = cnt + 1;
+            int P = 1;
+            while (P < L) P <<= 1;
+            for (int k = L + lane; k < P; k += 64) row[k] = 0x7fffffff;
+            __builtin_amdgcn_wave_barrier();
+            for (int k = 2; k <= P; k <<= 1)
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int a = lane; a < P; a += 64) {
+                        const int b2 = a ^ j;
+                        if (b2 > a) {
+                            const int x = row[a], y = row[b2];
+                            const bool up = (a & k) == 0;
+                            if (up ? x > y : x < y) {
+                                row[a] = y;
+                                row[b2] = x;
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            const long long start = ptr[pidx[i]];
+            for (int k = lane; k < L; k += 64) ind[start + k] = row[k];
+            __builtin_amdgcn_wave_barrier();                 // the row's reads precede the next row's writes
+        }
+    }
+}
+
+}  // namespace mph

@@ -57,6 +57,7 @@ def load_library() -> C.CDLL:
         "mph_get_fields": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(MphFields)]),
         "mph_compute_virial": (C.c_int, [vp]),
         "mph_get_neighbors": (C.c_int, [vp, C.c_int, vp, vp]),
+        "mph_get_neighbors_noalloc": (C.c_int, [vp, C.c_int, vp, vp, C.c_longlong, vp]),
         "mph_get_stats": (C.c_int, [vp, C.POINTER(MphStats)]),
         "mph_destroy": (None, [vp]),
         "mph_last_error": (C.c_char_p, []),
