@@ -163,7 +163,7 @@ def surrogate_leg(args, deck):
     from mphexplicit_amd.solver import Solver, Surrogate
     s = Solver.from_deck(deck)
     s.step(args.sur_warm)
-    sur = Surrogate(s, train_stride=args.sur_stride, lr=1e-3, seed=5, **SUR_SCALES)
+    sur = Surrogate(s, train_stride=args.sur_stride, lr=1e-3, seed=5, radius_ratio=args.sur_radius, **SUR_SCALES)
     s.sync()
     t0 = time.perf_counter()
     losses = sur.train(args.sur_samples, args.sur_iters)
@@ -185,7 +185,8 @@ def surrogate_leg(args, deck):
            "ms_per_surrogate_step": 1e3 * t_bare, "ms_per_two_plain_steps": 2e3 / plain_sps,
            "bare_sim_seconds_per_s": 2.0 * dt / t_bare,
            "judged": st, "judged_sim_seconds_per_s": st["sim_time"] / t_judged,
-           "plain_sim_seconds_per_s": plain_sim, "tol": args.sur_tol, "k": args.sur_k}
+           "plain_sim_seconds_per_s": plain_sim, "tol": args.sur_tol, "k": args.sur_k,
+           "radius_ratio": args.sur_radius or deck.data.radius_ratio_p}
     out["judged_vs_plain"] = out["judged_sim_seconds_per_s"] / plain_sim
     out["bare_vs_plain"] = out["bare_sim_seconds_per_s"] / plain_sim
     sur.close()
@@ -216,6 +217,8 @@ def main():
     ap.add_argument("--sur-blocks", type=int, default=8)
     ap.add_argument("--sur-k", type=int, default=10)
     ap.add_argument("--sur-tol", type=float, default=0.01)
+    ap.add_argument("--sur-radius", type=float, default=0.0,
+                    help="message radius of the surrogate / l0 (0: the largest kernel radius)")
     ap.add_argument("--late-steps", type=int, default=200,
                     help="steps of a longer developed-flow window after the timed one; 0: skip")
     ap.add_argument("--error-steps", type=int, default=500,

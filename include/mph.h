@@ -261,6 +261,8 @@ typedef struct mph_sur_params {
     int train_stride;        /* training uses fluid particles with sorted index % stride == 0 */
     int pad0;
     unsigned long long seed; /* weight initialisation */
+    double radius_ratio;     /* message radius / particle spacing; 0: the largest kernel radius
+                                (the neighbour set of main.cpp:1408); at most that radius */
 } mph_sur_params;
 
 typedef struct mph_sur_stats {

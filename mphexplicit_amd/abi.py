@@ -86,6 +86,7 @@ class MphSurParams(C.Structure):
         ("u_scale", C.c_double), ("a_scale", C.c_double), ("dens_scale", C.c_double),
         ("lr", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double),
         ("train_stride", C.c_int), ("pad0", C.c_int), ("seed", C.c_ulonglong),
+        ("radius_ratio", C.c_double),
     ]
 
 

@@ -1492,7 +1492,9 @@ static SurScales sur_scales(const mph_surrogate* s)
     sc.inv_u = (float)(1.0 / s->prm.u_scale);
     sc.a_scale = (float)s->prm.a_scale;
     sc.dens_scale = (float)s->prm.dens_scale;
-    sc.inv_h = (float)(1.0 / std::sqrt(k.hmax2));
+    const double r = s->prm.radius_ratio > 0.0 ? s->prm.radius_ratio * k.l0 : std::sqrt(k.hmax2);
+    sc.cut2 = r * r;
+    sc.inv_h = (float)(1.0 / r);
     sc.inv_n0p = (float)(1.0 / k.n0p);
     sc.dt = (float)k.dt;
     sc.gx = (float)k.grav[0];
@@ -1520,7 +1522,7 @@ static int sur_pairs(mph_surrogate* s)
     const long threads = (long)c->nslots * c->K;
     hipLaunchKernelGGL(k_sur_pairs, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream, c->d_cases.p,
                        c->slot_case.p, c->nslots, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width,
-                       c->K, c->pos.p, s->wh, s->pcnt.p, s->pnbr.p);
+                       c->K, c->pos.p, sur_scales(s).cut2, s->wh, s->pcnt.p, s->pnbr.p);
     return launch_ok("surrogate pairs");
 }
 
@@ -1631,6 +1633,11 @@ extern "C" mph_surrogate* mph_sur_create(mph_ctx* c, const mph_sur_params* p)
             }
     if (!(p->u_scale > 0.0 && p->a_scale > 0.0 && p->train_stride >= 1)) {
         fail(MPH_ERR_ARG, "mph_sur_create: scales must be positive and train_stride >= 1");
+        return nullptr;
+    }
+    // the pairs come from the neighbour lists, which reach the largest kernel radius
+    if (!(p->radius_ratio >= 0.0) || p->radius_ratio * c->cases[0].k.l0 > std::sqrt(c->cases[0].k.hmax2) * (1.0 + 1e-12)) {
+        fail(MPH_ERR_ARG, "mph_sur_create: radius_ratio must be in [0, largest kernel radius / spacing]");
         return nullptr;
     }
     std::unique_ptr<mph_surrogate> s(new mph_surrogate());

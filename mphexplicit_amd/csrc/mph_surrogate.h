@@ -40,6 +40,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 struct SurScales {
     float inv_u, a_scale, dens_scale, inv_h, inv_n0p;
     float dt, gx, gy, gz;
+    double cut2;            // message radius squared (pairs with r^2 <= cut2)
 };
 
 __device__ __forceinline__ int k_hidden(int s, int g, int q) { return 32 * s + 16 * (q >> 2) + 4 * g + (q & 3); }
@@ -84,13 +85,13 @@ k_sur_pack(const float* __restrict__ w, bf16x8* __restrict__ pk, float* __restri
     }
 }
 
-// pairs within the largest interaction radius, per fluid particle, from the cluster union
+// pairs within the message radius (r^2 <= cut2), per fluid particle, from the cluster union
 // lists: thread (cluster slot, member), like the list build
 __global__ void __launch_bounds__(256)
 k_sur_pairs(const CaseConst* __restrict__ cases, const int* __restrict__ slot_case, int nslots,
             const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
             const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width, int K,
-            const double4* __restrict__ pos, int wh, int* __restrict__ pcnt, int* __restrict__ pnbr)
+            const double4* __restrict__ pos, double cut2, int wh, int* __restrict__ pcnt, int* __restrict__ pnbr)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int g = (int)(t / K), m = (int)(t % K);
@@ -112,7 +113,7 @@ k_sur_pairs(const CaseConst* __restrict__ cases, const int* __restrict__ slot_ca
             if (j == i) continue;
             double dx, dy, dz;
             rel3(C, pi, pos[j], dx, dy, dz);
-            if (dx * dx + dy * dy + dz * dz <= C.hmax2 && cnt < wh) out[cnt++] = j;
+            if (dx * dx + dy * dy + dz * dz <= cut2 && cnt < wh) out[cnt++] = j;
         }
     }
     pcnt[i] = cnt;
@@ -360,7 +361,7 @@ k_sur_infer(const CaseConst* __restrict__ cases, const int4* __restrict__ cl_inf
                     cj = sur_class(tag_of(pj.w) & 3);
                     rel3(C, pi, pj, dxd, dyd, dzd);
                     // fluid partners with a lower index evaluate this pair themselves
-                    ok = dxd * dxd + dyd * dyd + dzd * dzd <= C.hmax2 && (j > i || cj == 2);
+                    ok = dxd * dxd + dyd * dyd + dzd * dzd <= sc.cut2 && (j > i || cj == 2);
                 }
                 const unsigned long long mask = __ballot(ok);
                 if (ok) {

@@ -129,7 +129,14 @@ def _errors(a, b):
     return float(ev), float(en)
 
 
-def test_judge_and_gain(mods):
+def test_network_against_plain_2dt(mods):
+    """What the trained network gains over the method without it.  A surrogate step
+    advances 2 dt; without the network the same 2 dt steps are plain MPH steps at 2 dt
+    (mph_set_time_step).  Both start from the same developed state and are compared with
+    plain steps at dt after 2M dt.  Measured on this deck: the trained network is far
+    closer than the untrained one and its judged blocks are kept at the 1% bound, but plain
+    steps at 2 dt are more accurate still (and cheaper: the bench's surrogate leg), so the
+    network gains nothing over them here; the assertions state that measured ordering."""
     deck = decks.small_dam_break_3d()
     # trained on the window that is judged (from step 100, like the bench): its per-step
     # error measures 0.8-1.0% there, trained from rest 0.95-1.16%, at the judge's 1% bound
@@ -158,15 +165,11 @@ def test_judge_and_gain(mods):
         sur = mods.Surrogate(s, seed=5, **SCALES)
         if weights is not None:
             sur.set_weights(weights)
-        try:
-            sur.step(M)
-            results[label] = _errors(s.fields(names=names), fr)
-            assert abs(s.time() - ref.time()) <= 1e-12 * ref.time()
-        except mods.MphError as e:
-            # an unjudged run that compresses the fluid overflows a neighbour list: the
-            # library reports it instead of continuing with a wrong list
-            print(label, "diverged:", e, flush=True)
-            results[label] = (np.inf, np.inf)
+        sur.step(M)
+        # an untrained network can compress the fluid until the lists widen: the run goes on
+        ev, en = _errors(s.fields(names=names), fr)
+        results[label] = (float(np.nan_to_num(ev, nan=np.inf)), float(np.nan_to_num(en, nan=np.inf)))
+        assert abs(s.time() - ref.time()) <= 1e-12 * ref.time()
         s2 = start()
         sur2 = mods.Surrogate(s2, seed=5, **SCALES)
         if weights is not None:
@@ -176,14 +179,17 @@ def test_judge_and_gain(mods):
         print(label, "judged", st, flush=True)
         f2 = s2.fields(names=names)
         assert np.all(np.isfinite(f2["velocity"]))       # rollbacks keep the state sane
-    print("relative L2 (velocity, density) after %d dt from step %d: %s" % (
+    print("relative L2 (velocity, density) against plain dt after %d dt from step %d: %s" % (
         2 * M, W, {k: v for k, v in results.items() if not k.endswith("judged")}), flush=True)
-    tr, un = results["trained"], results["untrained"]
+    tr, un, p2e = results["trained"], results["untrained"], results["plain_2dt"]
     # the trained network is closer to the plain run than the untrained one
     assert tr[0] < un[0] and tr[1] < un[1]
     # the judge rejects every block of the untrained network and keeps blocks of the trained one
     assert results["untrained_judged"]["rollbacks"] == 3
     assert results["trained_judged"]["sur_steps"] > 0
+    # the method without the network, plain steps at 2 dt, is the more accurate one here
+    assert p2e[0] < tr[0] and p2e[1] < tr[1]
+    assert p2e[0] < 0.01 and p2e[1] < 0.01
 
 
 def test_judge_rolls_back_nan_weights(mods):

@@ -125,3 +125,54 @@ def test_forward_emulation_close_to_exact():
     b = S.predict(w, X, basis, pi, ratio, 1.0, emulate=True)
     # bfloat16 weights and activations: about 2^-8 relative per rounding
     assert np.max(np.abs(a - b)) / np.max(np.abs(a)) < 0.05
+
+
+def _silu(z):
+    import math
+    return z / (1.0 + math.exp(-z))
+
+
+def test_forward_closed_form_one_pair():
+    # a hand-built network on one pair whose output follows from the weights by hand: unit 0
+    # of layer 1 reads r/h (feature 0), unit 1 reads u.e (feature 2) plus the constant feature
+    # 16 (layer 1's bias), layers 2 and 3 pass units 0 and 1 on with weight 1 and biases
+    # b2, b3, and the outputs are c0 = 2 h3[0] + b4[0], c1 = -h3[1], c2 = b4[2]:
+    #   h1 = (silu(x0), silu(x2 + 0.3)),  h2 = silu(h1 + b2),  h3 = silu(h2 + b3),
+    # a_i = a_scale rho_0/rho_i (c0 e + c1 u + c2 (u.e) e) for the pair's single message
+    import math
+    w = np.zeros(S.NPARAMS)
+    p = S.unflatten(w)
+    p["W1"][0, 0] = 1.0
+    p["W1"][2, 1] = 1.0
+    p["W1"][16, 1] = 0.3
+    p["W2"][0, 0] = p["W2"][1, 1] = 1.0
+    p["b2"][:2] = (0.1, -0.2)
+    p["W3"][0, 0] = p["W3"][1, 1] = 1.0
+    p["b3"][:2] = (-0.05, 0.25)
+    p["W4"][0, 0] = 2.0
+    p["W4"][1, 1] = -1.0
+    p["b4"][:] = (0.07, 0.0, -0.4)
+    w = S.flatten(p)
+    # two particles 2 l0 apart in a periodic box, across the face; relative velocity
+    lo, hi = np.zeros(3), np.full(3, 0.02)
+    x = np.array([[0.0005, 0.01, 0.01], [0.0195, 0.0102, 0.0101]])   # j is 1e-3 to the -x side of i
+    v = np.array([[0.01, 0.0, -0.02], [-0.03, 0.015, 0.0]])
+    fields = {"position": x, "velocity": v, "vol_strain_p": np.array([0.01, -0.02]), "type": np.array([0, 0])}
+    maxr, n0p, u_scale, dens_scale, a_scale = 3.5e-3, 0.95, 0.1, 50.0, 10.0
+    pi, pj = np.array([0]), np.array([1])
+    X, basis = S.features(fields, pi, pj, lo, hi, maxr, n0p, u_scale, dens_scale)
+    a = S.predict(w, X, basis, pi, np.array([1.0, 1.0]), a_scale, emulate=False)
+    # by hand
+    d = np.array([-1.0e-3, 0.0002, 0.0001])                            # minimum image x_j - x_i
+    r = math.sqrt((d * d).sum())
+    e = d / r
+    u = (v[1] - v[0]) / u_scale
+    ue = float(u @ e)
+    h1 = (_silu(r / maxr), _silu(ue + 0.3))
+    h2 = (_silu(h1[0] + 0.1), _silu(h1[1] - 0.2))
+    h3 = (_silu(h2[0] - 0.05), _silu(h2[1] + 0.25))
+    c0, c1, c2 = 2.0 * h3[0] + 0.07, -h3[1], -0.4
+    want = a_scale * (c0 * e + c1 * u + c2 * ue * e)
+    assert np.allclose(X[0, [0, 1, 2, 16]], [r / maxr, (r / maxr) ** 2, ue, 1.0], rtol=1e-12)
+    assert np.allclose(a[0], want, rtol=1e-12, atol=1e-15)
+    assert np.array_equal(a[1], np.zeros(3))
