@@ -253,8 +253,8 @@ def main():
                 "kernel": "k_" + dom,
                 "note": "FP64 VALU kernel; MI355X's fp64 vector and matrix peaks are both 78.6 TFLOP/s; "
                         "achieved = %d algorithmic flops/pair x %.4g pairs per launch / mean launch time; "
-                        "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by a "
-                        "copy probe), profiles/hbm_traffic_%s.json" % (fa if dom == "sweep_a" else fb,
+                        "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by the "
+                        "counters of a device copy of known size), profiles/hbm_traffic_%s.json" % (fa if dom == "sweep_a" else fb,
                                                                       pairs_per_step, deck.name)}
     mixed = None
     if prec == MPH_FP64 and not args.no_mixed:
