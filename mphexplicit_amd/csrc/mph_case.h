@@ -22,6 +22,7 @@ struct CaseConst {
     int cl_base, cl_cap;    // this case's cluster slots [cl_base, cl_base + cl_cap)
     int img_axes;           // bit d: images near a cluster's first member are exact on axis d
     int list_plain;         // the list build may decide pairs away from the faces by plain differences
+    int list_img;           // ... and near the faces by plain differences of images (on img_axes)
     double dmin[3], dmax[3], dwid[3], hwid[3];   // domain min, max, width, 0.5*width
     double guard[3];        // distance from a face beyond which no list partner wraps
     double cw[3];                       // cell widths

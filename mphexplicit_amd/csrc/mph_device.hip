@@ -340,10 +340,17 @@ static int derive_case(const mph_params& p, int begin, int end, int nfluid, int 
         if (k.hwid[d] > k.cw[d] + reach) k.img_axes |= 1 << d;
     // plain differences round within 2^-52 (W/2 + cut-off) of the reference's periodic form,
     // under 1e-9 of the cut-off for W/2 below 1e6 cut-offs: the list build's plain test
-    // redoes the reference's arithmetic for q^2 within 1e-9 of the cut-off
+    // redoes the reference's arithmetic for q^2 within 1e-8 of the cut-off.  Images x + kW
+    // round within 2^-53 M more (M: the largest coordinate magnitude plus W); below
+    // 4e6 cut-offs q^2 of image differences stays within 7e-9 of the reference's
     k.list_plain = 1;
-    for (int d = 0; d < 3; ++d)
+    k.list_img = 1;
+    for (int d = 0; d < 3; ++d) {
         if (!(k.hwid[d] <= 1.0e6 * (maxr + margin))) k.list_plain = 0;
+        const double m = std::max(std::fabs(k.dmin[d]), std::fabs(k.dmax[d])) + k.dwid[d];
+        if (!(m <= 4.0e6 * (maxr + margin))) k.list_img = 0;
+    }
+    if (!k.list_plain) k.list_img = 0;
     k.hmax2 = maxr * maxr;
     k.shift2 = 0.5 * margin * 0.5 * margin;                             // main.cpp:1102
     return MPH_OK;
@@ -1278,7 +1285,6 @@ extern "C" int mph_get_neighbors(mph_ctx* c, int case_id, long long* ptr, int* i
             const int o = idx[il];
             if (nc[il] == 0) continue;
             std::vector<int> row;
-            row.push_back(o);
             for (int e = 0; e < len[g]; ++e) {
                 const int jl = lists[(size_t)g * c->width + e] - b;
                 if (jl < 0 || jl >= m) return fail(MPH_ERR_STATE, "neighbour index outside its case");
@@ -1288,9 +1294,13 @@ extern "C" int mph_get_neighbors(mph_ctx* c, int case_id, long long* ptr, int* i
                 const double dz = host_rel(sn[jl].z, sn[il].z, k.hwid[2], k.dwid[2]);
                 if (dx * dx + dy * dy + dz * dz <= k.cut2) row.push_back(idx[jl]);
             }
+            // the particle itself, once per visit of its own cell (more than once where the
+            // stencil repeats cells on an axis of fewer than three)
+            const long long nself = rowlen[o] - (long long)row.size();
+            if (nself < 1)
+                return fail(MPH_ERR_STATE, "neighbour row length mismatch (%zu others vs %lld)", row.size(), rowlen[o]);
+            row.insert(row.end(), (size_t)nself, o);
             std::sort(row.begin(), row.end());
-            if ((long long)row.size() != rowlen[o])
-                return fail(MPH_ERR_STATE, "neighbour row length mismatch (%zu vs %lld)", row.size(), rowlen[o]);
             std::copy(row.begin(), row.end(), ind + ptr[o]);
         }
     }
@@ -1369,7 +1379,7 @@ extern "C" int mph_get_neighbors_noalloc(mph_ctx* c, int case_id, long long* ptr
         return fail(MPH_ERR_ARG, "mph_get_neighbors_noalloc: ind holds %lld entries, the lists need %lld",
                     ind_capacity, tot);
     int pad = 1;
-    while (pad < c->width + 1) pad <<= 1;
+    while (pad < c->width + 32) pad <<= 1;      // the union's entries and up to 27 of the particle itself
     const int waves = std::max(1, std::min(4, (int)((64 * 1024) / ((size_t)pad * sizeof(int)))));
     int ncl = 0;
     HIP_TRY(hipMemcpyAsync(c->h_scalar, c->cl_count.p + case_id, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -25,6 +25,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <type_traits>
+
 #include "mph_case.h"
 
 namespace mph {
@@ -42,6 +44,11 @@ constexpr int kPairSlots = 1024;
 constexpr int kPairStride = 16;
 
 __device__ __forceinline__ int tag_of(double w) { return (int)__double_as_longlong(w); }
+
+// native 4-wide vectors: records built from them stay in registers (the HIP double4
+// wrapper type, copied as a whole, can keep a record in private memory)
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
 
 // Mod(a, w) = a - w*floor(a/w) of main.cpp:70, returning the identical double.
 // For 0 <= a < w the IEEE quotient a/w rounds to a value below 1, so floor gives 0 and
@@ -96,6 +103,15 @@ __device__ __forceinline__ double image_near(double x, double xf, double hw, dou
 {
     const double a = x - xf + hw;
     return a < 0.0 ? x + w : (a >= w ? x - w : x);
+}
+
+// a position's images near xf on the given axes
+__device__ __forceinline__ void image_axes(const CaseConst& C, int axes, const double4& xf, double& x, double& y,
+                                           double& z)
+{
+    if (axes & 1) x = image_near(x, xf.x, C.hwid[0], C.dwid[0]);
+    if (axes & 2) y = image_near(y, xf.y, C.hwid[1], C.dwid[1]);
+    if (axes & 4) z = image_near(z, xf.z, C.hwid[2], C.dwid[2]);
 }
 
 // For a particle far enough from the periodic faces that no list partner wraps, the pair
@@ -575,15 +591,40 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
 // (main.cpp:1329-1596 per member, united in ascending index).  The candidates are the
 // particles of the cell's 3^d stencil cells in ascending rank and index.  The wavefront takes
 // 64 at a time (one load per lane, the next block's loads in flight), drops those farther than
-// the cut-off from the members' bounding box (conservatively: cut-off^2 (1 + 1e-9) against a
-// box built from the same exact differences), compacts the rest into LDS and tests them as
-// rows (members) x slots with the reference's cut-off test.  A candidate joins a cluster's
-// union if one of its members has it within the cut-off and is not it; the union keeps
-// candidate order.  Per member: NeighborCount as the reference counts it (main.cpp:1408-1420).
+// the cut-off from the members' bounding box (conservatively: cut-off^2 (1 + 1e-8) against a
+// box built from the same differences as the pair test), and appends the rest to a ring of
+// 128 entries in LDS.  Every 64 ring entries are tested as rows (members) x slots with the
+// reference's cut-off test, and the block's union is appended to the list in one pass.  A
+// candidate joins a cluster's union if one of its members has it within the cut-off and is
+// not it; the union keeps candidate order.  Per member: NeighborCount as the reference counts
+// it (main.cpp:1408-1420).
 // Wavefronts loop over the cluster slots: a step without a rebuild costs a small grid, and a
 // rebuild of a few clusters runs them in parallel.  One thread per member walking the
 // candidates with a dependent load each measured 5.5-6 ms per full rebuild of the
-// 1.27M-particle deck (this kernel: 1.9 ms) and was latency-bound when few clusters rebuild.
+// 1.27M-particle deck; testing each 64-candidate block as soon as it was compacted (partly
+// filled row x slot steps, the union appended per step), 1.84 ms.
+//
+// Pair decisions.  A wavefront whose members are all at least `guard` inside both faces of
+// an axis has no list pair that wraps on that axis.  On such axes, and on the axes near a face
+// where the images near the first member are the minimum images (img_axes), pairs are decided
+// by the plain difference of the (image) coordinates; where that q^2 is within 1e-8 of the
+// cut-off, where it can round apart from the reference's periodic form, the lane redoes the
+// reference's arithmetic on the original positions.  list_img (host) bounds the rounding of
+// both forms below that band.  Other wavefronts use the reference's arithmetic per pair.
+
+// Ring entry of a candidate that passed the box test: x, y in one plane, z and the index in
+// the other (two 16-byte reads per test)
+struct alignas(16) ListCandZ {
+    double z;
+    int j;          // particle index
+    int fluid;      // 1: in the fluid block of its cell
+};
+
+struct ListLds {
+    v2d xy[128];
+    ListCandZ zj[128];
+    int srank[32], clo[32], cmid[32], cpre[33];
+};
 
 // P clusters of one cell per wavefront (P = 2: rows 0..K-1 are the members of cluster g,
 // rows K..2K-1 those of cluster g1): the candidates of the shared stencil are loaded, located
@@ -593,9 +634,7 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
                                                  const double4* __restrict__ pos, const int* __restrict__ start,
                                                  const int* __restrict__ rank, int width, int* __restrict__ cl_nbr,
                                                  int* __restrict__ cl_len, int* __restrict__ selfin,
-                                                 int* __restrict__ count_out, int* __restrict__ max_len,
-                                                 double4* cand, int* candj, int* candf, int* srank, int* clo,
-                                                 int* cmid, int* cpre)
+                                                 int* __restrict__ count_out, ListLds& L)
 {
     constexpr int S = 64 / (K * P);
     const int lane = threadIdx.x & 63;
@@ -605,194 +644,243 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
     const int4 inf = second ? info1 : info;
     const bool mine = mrow < inf.y;
     const int i = inf.x + (mine ? mrow : 0);
-    const double4 pi = pos[i];
-    const bool wall_i = (tag_of(pi.w) & 3) >= 2;
-    // A cluster whose members are all at least `guard` inside both faces of every axis has
-    // no list pair that wraps (the partners across a face are farther than the guard, which
-    // exceeds the cut-off, in the plain difference too).  Such a wavefront (plain, uniform)
-    // decides pairs by the plain difference, and redoes the reference's periodic arithmetic
-    // only for q^2 within 1e-9 of the cut-off, where the two can round apart.
-    const bool nearf = !(pi.x - C.dmin[0] >= C.guard[0] && C.dmax[0] - pi.x >= C.guard[0] &&
-                         pi.y - C.dmin[1] >= C.guard[1] && C.dmax[1] - pi.y >= C.guard[1] &&
-                         pi.z - C.dmin[2] >= C.guard[2] && C.dmax[2] - pi.z >= C.guard[2]);
-    const bool plain = C.list_plain && __ballot(nearf) == 0ull;
-    // members' bounding box, relative to the first member
+    const double4 pio = pos[i];
+    const bool wall_i = (tag_of(pio.w) & 3) >= 2;
     const double4 pf = pos[info.x];
+    const bool nx = !(pio.x - C.dmin[0] >= C.guard[0] && C.dmax[0] - pio.x >= C.guard[0]);
+    const bool ny = !(pio.y - C.dmin[1] >= C.guard[1] && C.dmax[1] - pio.y >= C.guard[1]);
+    const bool nz = !(pio.z - C.dmin[2] >= C.guard[2] && C.dmax[2] - pio.z >= C.guard[2]);
+    const int ax = (__ballot(nx) != 0ull ? 1 : 0) | (__ballot(ny) != 0ull ? 2 : 0) | (__ballot(nz) != 0ull ? 4 : 0);
+    const bool exact = ax == 0 ? !C.list_plain : !(C.list_img && (ax & ~C.img_axes) == 0);   // uniform
+    const bool img = !exact && ax != 0;
+    // the row's position: its images near the first member on the axes near a face
+    double xi = pio.x, yi = pio.y, zi = pio.z;
+    if (img) image_axes(C, ax, pf, xi, yi, zi);
+    // members' bounding box relative to the first member; the S slots of a row hold the same
+    // member, so the rows are reduced over the lane offsets S..32
     double bx0, by0, bz0;
-    if (plain) {
-        bx0 = pi.x - pf.x;
-        by0 = pi.y - pf.y;
-        bz0 = pi.z - pf.z;
+    if (exact) {
+        bx0 = rel_exact(pio.x, pf.x, C.hwid[0], C.dwid[0]);
+        by0 = rel_exact(pio.y, pf.y, C.hwid[1], C.dwid[1]);
+        bz0 = rel_exact(pio.z, pf.z, C.hwid[2], C.dwid[2]);
     } else {
-        bx0 = rel_exact(pi.x, pf.x, C.hwid[0], C.dwid[0]);
-        by0 = rel_exact(pi.y, pf.y, C.hwid[1], C.dwid[1]);
-        bz0 = rel_exact(pi.z, pf.z, C.hwid[2], C.dwid[2]);
+        bx0 = xi - pf.x;
+        by0 = yi - pf.y;
+        bz0 = zi - pf.z;
     }
     double bx1 = bx0, by1 = by0, bz1 = bz0;
-    for (int off = 32; off > 0; off >>= 1) {
+    for (int off = 32; off >= S; off >>= 1) {
         bx0 = fmin(bx0, __shfl_xor(bx0, off)); bx1 = fmax(bx1, __shfl_xor(bx1, off));
         by0 = fmin(by0, __shfl_xor(by0, off)); by1 = fmax(by1, __shfl_xor(by1, off));
         bz0 = fmin(bz0, __shfl_xor(bz0, off)); bz1 = fmax(bz1, __shfl_xor(bz1, off));
     }
-    const double bound = C.cut2 * (1.0 + 1e-9);
+    const double band = 1.0e-8 * C.cut2;
+    const double bound = C.cut2 + band;
     // the box test needs one minimum image for all members: |x_j - x_first| < W/2 on every
     // axis for candidates within the cut-off of a member (members share a cell narrower than
     // two cut-offs), which eight cut-offs per width guarantee; narrower boxes test everything
     const double w8 = 64.0 * C.cut2;
     const bool use_box = C.dwid[0] * C.dwid[0] >= w8 && C.dwid[1] * C.dwid[1] >= w8 &&
                          (C.dim == 2 || C.dwid[2] * C.dwid[2] >= w8);
-    auto near_box = [&](const double4& p) {
-        if (!use_box) return true;
-        double dx, dy, dz;
-        if (plain) {
-            dx = p.x - pf.x;
-            dy = p.y - pf.y;
-            dz = p.z - pf.z;
-        } else {
-            dx = rel_exact(p.x, pf.x, C.hwid[0], C.dwid[0]);
-            dy = rel_exact(p.y, pf.y, C.hwid[1], C.dwid[1]);
-            dz = rel_exact(p.z, pf.z, C.hwid[2], C.dwid[2]);
-        }
-        const double ex = fmax(fmax(dx - bx1, bx0 - dx), 0.0);
-        const double ey = fmax(fmax(dy - by1, by0 - dy), 0.0);
-        const double ez = fmax(fmax(dz - bz1, bz0 - dz), 0.0);
-        return ex * ex + ey * ey + ez * ez <= bound;
-    };
     // stencil cells: lane q < nc takes the q-th cell of the window, then its place in
-    // ascending rank
+    // ascending rank (repeated cells, on axes of fewer than 3 cells, in lane order)
     const int cid = info.z;
     const int iz = cid % C.cc[2];
     const int iy = (cid / C.cc[2]) % C.cc[1];
     const int ix = cid / (C.cc[1] * C.cc[2]);
     int wx[3], wy[3], wz[3];
-    const int nx = axis_window(ix, C.cc[0], 1, wx);
-    const int ny = axis_window(iy, C.cc[1], 1, wy);
-    const int nz = axis_window(iz, C.cc[2], C.dim == 2 ? 0 : 1, wz);
-    const int nc = nx * ny * nz;
+    const int ncx = axis_window(ix, C.cc[0], 1, wx);
+    const int ncy = axis_window(iy, C.cc[1], 1, wy);
+    const int ncz = axis_window(iz, C.cc[2], C.dim == 2 ? 0 : 1, wz);
+    const int nc = ncx * ncy * ncz;
     int v = 0;
     if (lane < nc) {
-        const int a = lane / (ny * nz), b = (lane / nz) % ny, d = lane % nz;
+        const int a = lane / (ncy * ncz), b = (lane / ncz) % ncy, d = lane % ncz;
         v = rank[C.rank_base + (wx[a] * C.cc[1] + wy[b]) * C.cc[2] + wz[d]];
-        srank[lane] = v;
+        L.srank[lane] = v;
     }
     __builtin_amdgcn_wave_barrier();
+    int ncell = 0;
     if (lane < nc) {
         int at = 0;
-        for (int q = 0; q < nc; ++q) at += srank[q] < v;
+        for (int q = 0; q < nc; ++q) {
+            const int u = L.srank[q];
+            at += u < v || (u == v && q < lane);
+        }
         const int k0 = C.key_base + 2 * v;
         const int lo = start[k0];
-        clo[at] = lo;
-        cmid[at] = start[k0 + 1];
-        cpre[at + 1] = start[k0 + 2] - lo;
+        L.clo[at] = lo;
+        L.cmid[at] = start[k0 + 1];
+        ncell = start[k0 + 2] - lo;
+        L.srank[at] = ncell;          // the counts in ascending rank (srank is read above)
     }
-    if (lane == 0) cpre[0] = 0;
     __builtin_amdgcn_wave_barrier();
-    if (lane == 0)
-        for (int q = 0; q < nc; ++q) cpre[q + 1] += cpre[q];
+    // exclusive prefix of the counts over the cells in ascending rank
+    int pre = lane < nc ? L.srank[lane] : 0;
+    for (int off = 1; off < 32; off <<= 1) {
+        const int t = __shfl_up(pre, off);
+        if (lane >= off) pre += t;
+    }
+    if (lane < nc) L.cpre[lane + 1] = pre;
+    if (lane == 0) L.cpre[0] = 0;
     __builtin_amdgcn_wave_barrier();
-    const int total = cpre[nc];
-    // candidate e -> particle index and fluid flag; the cells of one block are consecutive
-    // from cell q0
+    const int total = L.cpre[nc];
+    // a block's lanes e = base + lane take the cell q with cpre[q] <= e: from the block's first
+    // cell q0, the cells starting inside the block are applied in ascending order
     int q0 = 0;
-    auto locate = [&](int e, int& fl) {
-        int q = q0;
-        while (q + 1 < nc && cpre[q + 1] <= e) ++q;
-        const int j = clo[q] + (e - cpre[q]);
-        fl = j < cmid[q];
-        return j;
+    auto locate = [&](int base, int& j, int& fl) {
+        int off = L.clo[q0] - L.cpre[q0], mid = L.cmid[q0];
+        for (int q = q0 + 1; q < nc; ++q) {
+            const int b = L.cpre[q];
+            if (b >= base + 64) break;
+            if (base + lane >= b) {
+                off = L.clo[q] - b;
+                mid = L.cmid[q];
+            }
+            q0 = q;
+        }
+        j = base + lane + off;
+        fl = j < mid;
     };
     const unsigned long long below = (1ull << lane) - 1ull;
-    int count = 0, nfluid = 0, self = 0, ulen = 0, ulen1 = 0;
-    unsigned present = 0, present1 = 0;     // bit m: member m of the cluster is in its union list
+    int cnt = 0, fl_in = 0, ulen = 0, ulen1 = 0;
+    // per member m of each cluster, the number of its entries in the cluster's union (8 bits
+    // each; more than one only where a stencil repeats cells)
+    unsigned long long own = 0, own1 = 0;
     int* out = cl_nbr + (size_t)g * width;
     int* out1 = cl_nbr + (size_t)(P == 2 ? g1 : g) * width;
-    int fn = 0;
-    int jn = lane < total ? locate(lane, fn) : -1;
-    double4 pn = jn >= 0 ? pos[jn] : make_double4(0.0, 0.0, 0.0, 0.0);
-    for (int base = 0; base < total; base += 64) {
-        // compact this block's candidates near the box into LDS, in candidate order
-        const bool kp = jn >= 0 && near_box(pn);
-        const unsigned long long kb = __ballot(kp);
-        const int ns = (int)__popcll(kb);
-        __builtin_amdgcn_wave_barrier();
-        if (kp) {
-            const int at = (int)__popcll(kb & below);
-            cand[at] = pn;
-            candj[at] = jn;
-            candf[at] = fn;
-        }
-        __builtin_amdgcn_wave_barrier();
-        if (base + 64 < total) {
-            while (q0 + 1 < nc && cpre[q0 + 1] <= base + 64) ++q0;
-            const int e = base + 64 + lane;
-            jn = e < total ? locate(e, fn) : -1;
-            if (jn >= 0) pn = pos[jn];
-        }
-        const int nsteps = (ns + S - 1) / S;
-        for (int t = 0; t < nsteps; ++t) {
-            const int e = t * S + slot;
-            const int j = candj[e];
-            const double4 pj = cand[e];
-            double q2;
-            if (plain) {
-                const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                q2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                const bool edge = fabs(q2 - C.cut2) <= 1.0e-9 * C.cut2;
-                if (__ballot(edge) != 0ull && edge) {
-                    const double ex = rel_exact(pj.x, pi.x, C.hwid[0], C.dwid[0]);
-                    const double ey = rel_exact(pj.y, pi.y, C.hwid[1], C.dwid[1]);
-                    const double ez = rel_exact(pj.z, pi.z, C.hwid[2], C.dwid[2]);
-                    q2 = ex * ex + ey * ey + ez * ez;
+    // the walk over the candidates, specialised for the wavefront's path (EX: the reference's
+    // periodic arithmetic per pair)
+    auto walk = [&](auto ex_tag) {
+        constexpr bool EX = decltype(ex_tag)::value;
+        unsigned wp = 0, rp = 0;        // ring write and read counts (uniform)
+        // rows x slots over ring entries rp .. rp + nb - 1, then the block's union appended
+        auto test_block = [&](int nb) {
+            unsigned long long ub = 0, ub1 = 0;
+            const int nt = (nb + S - 1) / S;
+#pragma unroll 1
+            for (int t = 0; t < nt; ++t) {
+                const int e = t * S + slot;
+                const int s = (int)((rp + (unsigned)e) & 127u);
+                const v2d xy = L.xy[s];
+                const ListCandZ cz = L.zj[s];
+                double q2;
+                if constexpr (!EX) {
+                    const double dx = xy.x - xi, dy = xy.y - yi, dz = cz.z - zi;
+                    q2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                    // (slots past the block's last entry hold stale LDS: no load from them)
+                    const bool edge = e < nb && fabs(q2 - C.cut2) <= band;
+                    if (__ballot(edge) != 0ull && edge) {
+                        const double4 pj = pos[cz.j];
+                        const double ex = rel_exact(pj.x, pio.x, C.hwid[0], C.dwid[0]);
+                        const double ey = rel_exact(pj.y, pio.y, C.hwid[1], C.dwid[1]);
+                        const double ez = rel_exact(pj.z, pio.z, C.hwid[2], C.dwid[2]);
+                        q2 = ex * ex + ey * ey + ez * ez;
+                    }
+                } else {
+                    const double dx = rel_exact(xy.x, pio.x, C.hwid[0], C.dwid[0]);
+                    const double dy = rel_exact(xy.y, pio.y, C.hwid[1], C.dwid[1]);
+                    const double dz = rel_exact(cz.z, pio.z, C.hwid[2], C.dwid[2]);
+                    q2 = dx * dx + dy * dy + dz * dz;
                 }
-            } else {
-                const double dx = rel_exact(pj.x, pi.x, C.hwid[0], C.dwid[0]);
-                const double dy = rel_exact(pj.y, pi.y, C.hwid[1], C.dwid[1]);
-                const double dz = rel_exact(pj.z, pi.z, C.hwid[2], C.dwid[2]);
-                q2 = dx * dx + dy * dy + dz * dz;
+                const bool in = mine && e < nb && q2 <= C.cut2;
+                cnt += in;
+                fl_in |= in ? cz.fluid : 0;
+                const unsigned long long bv = __ballot(in && cz.j != i);
+                // OR of each cluster's K row chunks of S bits
+                constexpr unsigned long long smask = S == 64 ? ~0ull : ((1ull << S) - 1ull);
+                unsigned long long u = P == 2 ? (bv & 0xffffffffull) : bv, u1 = P == 2 ? (bv >> 32) : 0ull;
+                for (int sh = 32 / P; sh >= S; sh >>= 1) {
+                    u |= u >> sh;
+                    if (P == 2) u1 |= u1 >> sh;
+                }
+                ub |= (u & smask) << (t * S);
+                if (P == 2) ub1 |= (u1 & smask) << (t * S);
             }
-            const bool in = mine && e < ns && q2 <= C.cut2;
-            const bool other = in && j != i;
-            self |= in && j == i;
-            count += other;
-            nfluid += other && candf[e];
-            const unsigned long long bv = __ballot(other);
-            // OR of each cluster's K row chunks of S bits, folded in log2(K) halvings
-            constexpr unsigned long long smask = S == 64 ? ~0ull : ((1ull << S) - 1ull);
-            unsigned long long u = P == 2 ? (bv & 0xffffffffull) : bv, u1 = P == 2 ? (bv >> 32) : 0ull;
-            for (int sh = 32 / P; sh >= S; sh >>= 1) {
-                u |= u >> sh;
-                if (P == 2) u1 |= u1 >> sh;
-            }
-            u &= smask;
-            u1 &= smask;
-            if (row == 0 && ((u >> slot) & 1ull)) {
-                const int at = ulen + (int)__popcll(u & ((1ull << slot) - 1ull));
+            const int s = (int)((rp + (unsigned)lane) & 127u);
+            const int j = L.zj[s].j;
+            if ((ub >> lane) & 1ull) {
+                const int at = ulen + (int)__popcll(ub & below);
                 if (at < width) out[at] = j;
                 const unsigned m = (unsigned)(j - info.x);
-                if (m < (unsigned)info.y) present |= 1u << m;
+                if (m < (unsigned)info.y) own += 1ull << (8 * m);
             }
-            if (P == 2 && row == K && ((u1 >> slot) & 1ull)) {
-                const int at = ulen1 + (int)__popcll(u1 & ((1ull << slot) - 1ull));
-                if (at < width) out1[at] = j;
-                const unsigned m = (unsigned)(j - info1.x);
-                if (m < (unsigned)info1.y) present1 |= 1u << m;
+            ulen += (int)__popcll(ub);
+            if (P == 2) {
+                if ((ub1 >> lane) & 1ull) {
+                    const int at = ulen1 + (int)__popcll(ub1 & below);
+                    if (at < width) out1[at] = j;
+                    const unsigned m = (unsigned)(j - info1.x);
+                    if (m < (unsigned)info1.y) own1 += 1ull << (8 * m);
+                }
+                ulen1 += (int)__popcll(ub1);
             }
-            ulen += (int)__popcll(u);
-            if (P == 2) ulen1 += (int)__popcll(u1);
+            rp += (unsigned)nb;
+        };
+        int jn = -1, fn = 0;
+        q0 = 0;
+        if (total > 0) locate(0, jn, fn);
+        double4 pn = lane < total ? pos[jn] : make_double4(0.0, 0.0, 0.0, 0.0);
+        for (int base = 0; base < total; base += 64) {
+            // this block's candidates near the box, appended to the ring in candidate order
+            double px = pn.x, py = pn.y, pz = pn.z;
+            if (!EX && img) image_axes(C, ax, pf, px, py, pz);
+            bool kp = base + lane < total;
+            if (kp && use_box) {
+                double dx, dy, dz;
+                if constexpr (EX) {
+                    dx = rel_exact(px, pf.x, C.hwid[0], C.dwid[0]);
+                    dy = rel_exact(py, pf.y, C.hwid[1], C.dwid[1]);
+                    dz = rel_exact(pz, pf.z, C.hwid[2], C.dwid[2]);
+                } else {
+                    dx = px - pf.x;
+                    dy = py - pf.y;
+                    dz = pz - pf.z;
+                }
+                const double ex = fmax(fmax(dx - bx1, bx0 - dx), 0.0);
+                const double ey = fmax(fmax(dy - by1, by0 - dy), 0.0);
+                const double ez = fmax(fmax(dz - bz1, bz0 - dz), 0.0);
+                kp = fma(ez, ez, fma(ey, ey, ex * ex)) <= bound;
+            }
+            const unsigned long long kb = __ballot(kp);
+            if (kp) {
+                const int s = (int)((wp + (unsigned)__popcll(kb & below)) & 127u);
+                L.xy[s] = v2d{px, py};
+                ListCandZ cz;
+                cz.z = pz;
+                cz.j = jn;
+                cz.fluid = fn;
+                L.zj[s] = cz;
+            }
+            wp += (unsigned)__popcll(kb);
+            // the next block's loads are in flight while the ring is tested
+            if (base + 64 < total) {
+                locate(base + 64, jn, fn);
+                if (base + 64 + lane < total) pn = pos[jn];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (wp - rp >= 64u) test_block(64);
+            __builtin_amdgcn_wave_barrier();
         }
-    }
+        if (wp != rp) test_block((int)(wp - rp));
+    };
+    if (exact)
+        walk(std::true_type{});
+    else
+        walk(std::false_type{});
     for (int off = S / 2; off > 0; off >>= 1) {
-        count += __shfl_xor(count, off, S);
-        nfluid += __shfl_xor(nfluid, off, S);
-        self |= __shfl_xor(self, off, S);
+        cnt += __shfl_xor(cnt, off, S);
+        fl_in |= __shfl_xor(fl_in, off, S);
     }
     for (int off = 32; off > 0; off >>= 1) {
-        present |= __shfl_xor(present, off);
-        if (P == 2) present1 |= __shfl_xor(present1, off);
+        own += __shfl_xor(own, off);
+        if (P == 2) own1 += __shfl_xor(own1, off);
     }
-    if (mine && slot == 0) count_out[i] = (wall_i && nfluid == 0) ? 0 : count + self;
-    if (lane < info.y) selfin[info.x + lane] = (present >> lane) & 1u;
-    if (P == 2 && lane < info1.y) selfin[info1.x + lane] = (present1 >> lane) & 1u;
+    // NeighborCount: the member's own entry included; a wall particle with no fluid partner
+    // has none (main.cpp:1416)
+    if (mine && slot == 0) count_out[i] = (wall_i && fl_in == 0) ? 0 : cnt;
+    if (lane < info.y) selfin[info.x + lane] = (int)((own >> (8 * lane)) & 0xffull);
+    if (P == 2 && lane < info1.y) selfin[info1.x + lane] = (int)((own1 >> (8 * lane)) & 0xffull);
     if (lane == 0) cl_len[g] = ulen < width ? ulen : width;
     if (P == 2 && lane == 0) cl_len[g1] = ulen1 < width ? ulen1 : width;
     __builtin_amdgcn_wave_barrier();       // this cluster's LDS reads precede the next one's writes
@@ -811,10 +899,7 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
                 int* __restrict__ max_len)
 {
     if (no_rebuild(any_flag)) return;
-    __shared__ double4 cand[4][64];
-    __shared__ int candj[4][64];         // particle index
-    __shared__ int candf[4][64];         // 1: in the fluid block of its cell
-    __shared__ int srank[4][32], clo[4][32], cmid[4][32], cpre[4][33];
+    __shared__ ListLds lds[4];
     // the wavefront's slot and case are uniform: readfirstlane lets the compiler keep the
     // case's constants in scalar registers (scalar loads) instead of vector loads in the loop
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -822,8 +907,7 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
     const int4 none = make_int4(0, 0, 0, 0);
 #define MPH_LIST_ONE(PP, CC, G, I, G1, I1)                                                                  \
     cluster_list_one<K, PP>(cases[CC], G, I, G1, I1, pos, start, rank, width, cl_nbr, cl_len, selfin,       \
-                            count_out, max_len, cand[wv], candj[wv], candf[wv], srank[wv], clo[wv], cmid[wv], \
-                            cpre[wv])
+                            count_out, lds[wv])
     if (K <= 4) {
         // slots in pairs (2t, 2t + 1): two clusters of one cell share a wavefront
         for (int t = blockIdx.x * 4 + wv; 2 * t < nslots; t += gridDim.x * 4) {
@@ -934,11 +1018,6 @@ __device__ __forceinline__ void walk_union(const Frame<K>& f, Load load, Eval ev
         j1 = j3;
     }
 }
-
-// native 4-wide vectors: records built from them stay in registers (the HIP double4
-// wrapper type, copied as a whole, can keep a record in private memory)
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef double v2d __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ v4d ld4(const double4* p) { return *reinterpret_cast<const v4d*>(p); }
 
@@ -1067,15 +1146,6 @@ __device__ __forceinline__ int wrap_path(const CaseConst& C, const double4& p, i
     const bool nz = !(p.z - C.dmin[2] >= C.guard[2] && C.dmax[2] - p.z >= C.guard[2]);
     axes = (__ballot(nx) != 0ull ? 1 : 0) | (__ballot(ny) != 0ull ? 2 : 0) | (__ballot(nz) != 0ull ? 4 : 0);
     return axes == 0 ? 0 : ((axes & ~C.img_axes) == 0 ? 1 : 2);
-}
-
-// a position's images near xf on the given axes
-__device__ __forceinline__ void image_axes(const CaseConst& C, int axes, const double4& xf, double& x, double& y,
-                                           double& z)
-{
-    if (axes & 1) x = image_near(x, xf.x, C.hwid[0], C.dwid[0]);
-    if (axes & 2) y = image_near(y, xf.y, C.hwid[1], C.dwid[1]);
-    if (axes & 4) z = image_near(z, xf.z, C.hwid[2], C.dwid[2]);
 }
 
 template <bool ON>
@@ -1656,15 +1726,18 @@ k_csr_rows(const CaseConst* __restrict__ cases, int cc, const int4* __restrict__
                 }
                 const unsigned long long b = __ballot(in);
                 const int at = cnt + (int)__popcll(b & below);
-                if (in && at < pad - 1) row[at] = o;
+                if (in && at < pad) row[at] = o;
                 cnt += (int)__popcll(b);
             }
-            if (cnt + 1 != want) {                           // cannot happen with consistent lists
+            // the particle itself, once per visit of its own cell (more than once where the
+            // stencil repeats cells on an axis of fewer than three)
+            const int nself = want - cnt;
+            if (nself < 1 || cnt + nself > pad) {            // cannot happen with consistent lists
                 if (lane == 0) atomicAdd(err, 1);
                 continue;
             }
-            if (lane == 0) row[cnt] = pidx[i];               // self
-            const int L = cnt + 1;
+            for (int k = lane; k < nself; k += 64) row[cnt + k] = pidx[i];
+            const int L = cnt + nself;
             int P = 1;
             while (P < L) P <<= 1;
             for (int k = L + lane; k < P; k += 64) row[k] = 0x7fffffff;

@@ -468,6 +468,21 @@ def periodic_flow_3d(n_side=20, l0=1.0e-3, drift=(0.02, 0.05, 0.0), seed=16, ste
                 np.zeros(len(x), np.int32), x, v, steps=steps)
 
 
+def narrow_periodic_3d(shape=(16, 5, 8), l0=1.0e-3, seed=20, steps=20, name=None) -> Deck:
+    """Fluid filling a periodic box with axes narrower than three cells (here one cell on y and
+    two on z): the reference's stencil then visits cells more than once (main.cpp:1365-1391),
+    and every list holds such repeats."""
+    rng = np.random.default_rng(seed)
+    rho, mu = 1000.0, 1.0e-3
+    K, dt = sound_speed_params(rho, 1.0, l0)
+    x = _jitter(rng, _lattice(shape, l0, (0.0, 0.0, 0.0)), 0.02 * l0, 3)
+    v = rng.uniform(-0.02, 0.02, size=x.shape)
+    data = DataParams(dt=dt, end_time=dt * steps, density=(rho,) * 4, bulk_modulus=(K,) * 4,
+                      bulk_viscosity=(mu,) * 4, shear_viscosity=(mu,) * 4, surface_tension=(0.0,) * 4)
+    return Deck(name or "narrow_periodic3d", 3, data, l0, np.zeros(3), np.asarray(shape, np.float64) * l0,
+                np.zeros(len(x), np.int32), x, v, steps=steps)
+
+
 def sparse_random_3d(n=1500, side=24, l0=1.0e-3, seed=17, steps=10, name=None) -> Deck:
     """Particles at uniformly random positions and velocities in a periodic cube, about 0.11
     per l0^3 (a tenth of the lattice density), no gravity: cells hold a few particles far apart,
@@ -544,6 +559,31 @@ def small_moving_wall_2d(seed=14) -> Deck:
                       wall_center=wc, wall_velocity=wv, wall_omega=wo)
     L = 0.04
     return Deck("small_movingwall2d", 2, data, l0, np.array([-L, -L, -5e-4]), np.array([L, L, 5e-4]),
+                t, x, np.zeros((len(x), 3)), steps=50)
+
+
+def small_rotating_wall_3d(seed=19) -> Deck:
+    """3D fluid block on a translating plate (type 2) beside a wall (type 3); both walls
+    rotate about axes off the coordinate axes, so every component of the in-place rotation
+    r <- R r (main.cpp:2033-2035) reads an already rotated one."""
+    rng = np.random.default_rng(seed)
+    l0 = 1.0e-3
+    xf = _jitter(rng, _lattice((8, 8, 8), l0, (-4 * l0, 0.0, -4 * l0)), 0.01 * l0, 3)
+    xw2 = _lattice((14, 3, 14), l0, (-7 * l0, -3 * l0, -7 * l0))
+    xw3 = _lattice((3, 8, 14), l0, (-7 * l0, 0.0, -7 * l0))
+    x = np.concatenate([xf, xw2, xw3])
+    t = np.concatenate([np.zeros(len(xf), np.int32), np.full(len(xw2), 2, np.int32),
+                        np.full(len(xw3), 3, np.int32)])
+    K, dt = sound_speed_params(1000.0, 2.0, l0)
+    wc = ((0, 0, 0), (0, 0, 0), (0.0, -1.5e-3, 0.0), (-6e-3, 4e-3, 0.0))
+    wv = ((0, 0, 0), (0, 0, 0), (0.04, 0.0, -0.02), (0.0, 0.01, 0.0))
+    wo = ((0, 0, 0), (0, 0, 0), (0.3, -0.4, 0.5), (0.6, 0.2, -0.7))
+    data = DataParams(dt=dt, end_time=dt * 50, density=(1000.0,) * 4, bulk_modulus=(K,) * 4,
+                      bulk_viscosity=(0.05,) * 4, shear_viscosity=(0.05,) * 4,
+                      surface_tension=(0.05,) * 4, gravity=(0.0, -9.8, 0.0),
+                      wall_center=wc, wall_velocity=wv, wall_omega=wo)
+    L = 0.03
+    return Deck("small_rotatingwall3d", 3, data, l0, np.array([-L, -L, -L]), np.array([L, L, L]),
                 t, x, np.zeros((len(x), 3)), steps=50)
 
 

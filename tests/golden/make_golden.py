@@ -5,7 +5,7 @@ runs it through oracle.oracle.Reference on the small decks of mphexplicit_amd.de
 stores inputs-derived outputs as compressed .npz files (fields keyed by original particle
 id, neighbour lists by id, momentum histories).  Run from the repo root:
 
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py [name ...]
 """
 from __future__ import annotations
 
@@ -31,6 +31,7 @@ CASES = {
     "bridge2d": (decks.sample_bridge, False, 20),
     "dambreak3d": (decks.small_dam_break_3d, False, 10),
     "twofluid3d": (decks.small_two_fluid_3d, False, 10),
+    "rotatingwall3d": (decks.small_rotating_wall_3d, True, 50),
 }
 SAMPLE = 200
 
@@ -51,8 +52,9 @@ def momentum(fields):
     return (m * fields["v"]).sum(axis=0)
 
 
-def main():
-    for name, (factory, full, msteps) in CASES.items():
+def main(names):
+    for name in names or list(CASES):
+        factory, full, msteps = CASES[name]
         d = factory()
         ref = Reference(d, nthreads=os.cpu_count() or 1)
         rng = np.random.default_rng(2024)
@@ -86,4 +88,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -20,6 +20,8 @@ DECKS = {
     "bridge2d": decks.sample_bridge,
     "squaredrop2d": decks.sample_square_drop,
     "movingwall2d": decks.small_moving_wall_2d,
+    # walls rotating about axes off the coordinate axes (the in-place rotation in 3D)
+    "rotatingwall3d": decks.small_rotating_wall_3d,
     "dambreak3d": decks.small_dam_break_3d,
     "twofluid3d": decks.small_two_fluid_3d,
     # every width above eight cut-offs: the list build's bounding-box test is on in 3D
@@ -30,6 +32,9 @@ DECKS = {
     # a tenth of the lattice density at random positions: cluster members with no other
     # member within the cut-off (their own index is not in the union list)
     "sparse3d": decks.sparse_random_3d,
+    # periodic axes of one and two cells: the stencil visits cells more than once, and the
+    # lists repeat their entries (the particle itself included) as the reference's do
+    "narrowperiodic3d": decks.narrow_periodic_3d,
 }
 
 COMPARED = ("x", "v", "force", "dens_a", "vol_strain", "divergence", "press_p", "press_a", "gcenter")
@@ -114,7 +119,7 @@ def test_first_step_parity(solver_cls, name):
 
 
 @pytest.mark.parametrize("name", ["droplet2d", "movingwall2d", "bridge2d", "squaredrop2d", "twofluid3d", "wide3d",
-                                  "periodicflow3d"])
+                                  "periodicflow3d", "rotatingwall3d", "narrowperiodic3d"])
 def test_multi_step_parity(solver_cls, name):
     print("case", name, flush=True)
     d = DECKS[name]()
@@ -159,7 +164,7 @@ def test_deterministic(solver_cls, monkeypatch):
         assert np.array_equal(fa[k], fb[k]), k
 
 
-@pytest.mark.parametrize("name", ["droplet2d", "movingwall2d", "twofluid3d"])
+@pytest.mark.parametrize("name", ["droplet2d", "movingwall2d", "twofluid3d", "rotatingwall3d"])
 def test_virial_matches_oracle(solver_cls, name):
     # calculateVirialStressAtParticle (main.cpp:2057-2302) at an output step
     d = DECKS[name]()
@@ -175,7 +180,8 @@ def test_virial_matches_oracle(solver_cls, name):
     assert err <= FP64_TOL
 
 
-@pytest.mark.parametrize("name", ["droplet2d", "movingwall2d", "bridge2d", "dambreak3d", "twofluid3d"])
+@pytest.mark.parametrize("name", ["droplet2d", "movingwall2d", "bridge2d", "dambreak3d", "twofluid3d",
+                                  "rotatingwall3d"])
 def test_golden_fixtures(solver_cls, name):
     # the reference's own outputs (tests/golden, written from source/main.cpp)
     import os
@@ -200,7 +206,8 @@ def test_golden_fixtures(solver_cls, name):
 
 
 @pytest.mark.parametrize("K", ["2", "4", "8"])
-@pytest.mark.parametrize("name", ["twofluid3d", "movingwall2d", "wide3d", "periodicflow3d", "sparse3d"])
+@pytest.mark.parametrize("name", ["twofluid3d", "movingwall2d", "wide3d", "periodicflow3d", "sparse3d",
+                                  "rotatingwall3d", "narrowperiodic3d"])
 def test_every_cluster_size(solver_cls, name, K, monkeypatch):
     # each tile shape the autotuner can pick gives the reference's lists and fields
     monkeypatch.setenv("MPH_TUNE", K)

@@ -24,6 +24,7 @@ GOLDEN_DECKS = {
     "bridge2d": decks.sample_bridge,
     "dambreak3d": decks.small_dam_break_3d,
     "twofluid3d": decks.small_two_fluid_3d,
+    "rotatingwall3d": decks.small_rotating_wall_3d,
 }
 
 
@@ -65,7 +66,8 @@ def test_oracle_matches_golden(name):
 @pytest.mark.skipif(not (reference_available(2) and reference_available(3)),
                     reason="reference build oracle/_ref not present")
 @pytest.mark.parametrize("factory,steps", [(decks.small_droplet_2d, 40), (decks.sample_square_drop, 30),
-                                           (decks.small_moving_wall_2d, 40), (decks.small_two_fluid_3d, 6)])
+                                           (decks.small_moving_wall_2d, 40), (decks.small_two_fluid_3d, 6),
+                                           (decks.small_rotating_wall_3d, 40), (decks.narrow_periodic_3d, 20)])
 def test_oracle_bitwise_vs_reference_build(factory, steps):
     d = factory()
     o, r = Oracle(d, nthreads=4), Reference(d, nthreads=4)
