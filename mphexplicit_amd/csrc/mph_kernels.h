@@ -593,16 +593,16 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
 // 64 at a time (one load per lane, the next block's loads in flight), drops those farther than
 // the cut-off from the members' bounding box (conservatively: cut-off^2 (1 + 1e-8) against a
 // box built from the same differences as the pair test), and appends the rest to a ring of
-// 128 entries in LDS.  Every 64 ring entries are tested as rows (members) x slots with the
-// reference's cut-off test, and the block's union is appended to the list in one pass.  A
-// candidate joins a cluster's union if one of its members has it within the cut-off and is
-// not it; the union keeps candidate order.  Per member: NeighborCount as the reference counts
-// it (main.cpp:1408-1420).
+// 128 entries in LDS.  Every 64 ring entries are tested one entry per lane against each member
+// in turn (a member's hits are one ballot), and the block's union is appended to the list in
+// one pass.  A candidate joins a cluster's union if one of its members has it within the
+// cut-off and is not it; the union keeps candidate order.  Per member: NeighborCount as the
+// reference counts it (main.cpp:1408-1420).
 // Wavefronts loop over the cluster slots: a step without a rebuild costs a small grid, and a
-// rebuild of a few clusters runs them in parallel.  One thread per member walking the
-// candidates with a dependent load each measured 5.5-6 ms per full rebuild of the
-// 1.27M-particle deck; testing each 64-candidate block as soon as it was compacted (partly
-// filled row x slot steps, the union appended per step), 1.84 ms.
+// rebuild of a few clusters runs them in parallel.  Full rebuilds of the 1.27M-particle deck:
+// one thread per member walking the candidates with a dependent load each, 5.5-6 ms; rows
+// (members) x slots testing each 64-candidate block as soon as it was compacted (partly
+// filled steps, the union appended per step), 1.84 ms; this kernel, 1.19 ms.
 //
 // Pair decisions.  A wavefront whose members are all at least `guard` inside both faces of
 // an axis has no list pair that wraps on that axis.  On such axes, and on the axes near a face
@@ -612,23 +612,25 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
 // reference's arithmetic on the original positions.  list_img (host) bounds the rounding of
 // both forms below that band.  Other wavefronts use the reference's arithmetic per pair.
 
-// Ring entry of a candidate that passed the box test: x, y in one plane, z and the index in
-// the other (two 16-byte reads per test)
-struct alignas(16) ListCandZ {
-    double z;
-    int j;          // particle index
-    int fluid;      // 1: in the fluid block of its cell
-};
-
+// Ring of the candidates that passed the box test, in planes (x, y | z | index with the
+// fluid flag in bit 31), and the members of the wavefront's clusters (x, y, z, index): the
+// ring's planes are read and written at consecutive entries, the members by broadcast
 struct ListLds {
     v2d xy[128];
-    ListCandZ zj[128];
+    double z[128];
+    int jf[128];
+    double4 mem[8];
     int srank[32], clo[32], cmid[32], cpre[33];
 };
 
-// P clusters of one cell per wavefront (P = 2: rows 0..K-1 are the members of cluster g,
-// rows K..2K-1 those of cluster g1): the candidates of the shared stencil are loaded, located
-// and box-tested once for both, and each cluster's rows unite into its own list.
+// wave-wide ballot of a lane predicate straight into a scalar mask
+__device__ __forceinline__ unsigned long long ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// P clusters of one cell per wavefront (P = 2: member slots 0..K-1 are cluster g's, K..2K-1
+// cluster g1's): the candidates of the shared stencil are loaded, located and box-tested
+// once for both, and each cluster's members unite into its own list.  Each lane holds one
+// ring entry and tests it against the members in turn (their positions broadcast from LDS), so
+// a member's hits are one ballot and the union of a cluster the OR of its members' ballots.
 template <int K, int P>
 __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const int4 info, int g1, const int4 info1,
                                                  const double4* __restrict__ pos, const int* __restrict__ start,
@@ -636,28 +638,30 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
                                                  int* __restrict__ cl_len, int* __restrict__ selfin,
                                                  int* __restrict__ count_out, ListLds& L)
 {
-    constexpr int S = 64 / (K * P);
+    constexpr int M = K * P;                           // member slots
     const int lane = threadIdx.x & 63;
-    const int row = lane / S, slot = lane % S;
-    const bool second = P == 2 && row >= K;            // row of cluster g1
-    const int mrow = second ? row - K : row;
+    // lane l < M takes member slot l; the other lanes repeat the slots (l mod M), and slots
+    // of absent members repeat the cluster's first member (which does not change the box)
+    const int ms = lane % M;
+    const bool second = P == 2 && ms >= K;
     const int4 inf = second ? info1 : info;
-    const bool mine = mrow < inf.y;
-    const int i = inf.x + (mine ? mrow : 0);
+    const int mk = second ? ms - K : ms;
+    const bool present = mk < inf.y;
+    const int i = inf.x + (present ? mk : 0);
     const double4 pio = pos[i];
     const bool wall_i = (tag_of(pio.w) & 3) >= 2;
     const double4 pf = pos[info.x];
     const bool nx = !(pio.x - C.dmin[0] >= C.guard[0] && C.dmax[0] - pio.x >= C.guard[0]);
     const bool ny = !(pio.y - C.dmin[1] >= C.guard[1] && C.dmax[1] - pio.y >= C.guard[1]);
     const bool nz = !(pio.z - C.dmin[2] >= C.guard[2] && C.dmax[2] - pio.z >= C.guard[2]);
-    const int ax = (__ballot(nx) != 0ull ? 1 : 0) | (__ballot(ny) != 0ull ? 2 : 0) | (__ballot(nz) != 0ull ? 4 : 0);
+    const int ax = (ballot64(nx) != 0ull ? 1 : 0) | (ballot64(ny) != 0ull ? 2 : 0) | (ballot64(nz) != 0ull ? 4 : 0);
     const bool exact = ax == 0 ? !C.list_plain : !(C.list_img && (ax & ~C.img_axes) == 0);   // uniform
     const bool img = !exact && ax != 0;
-    // the row's position: its images near the first member on the axes near a face
+    // the member's position: its images near the first member on the axes near a face
     double xi = pio.x, yi = pio.y, zi = pio.z;
     if (img) image_axes(C, ax, pf, xi, yi, zi);
-    // members' bounding box relative to the first member; the S slots of a row hold the same
-    // member, so the rows are reduced over the lane offsets S..32
+    if (lane < M) L.mem[lane] = make_double4(xi, yi, zi, __longlong_as_double((long long)i));
+    // members' bounding box relative to the first member (reduced over the M slots)
     double bx0, by0, bz0;
     if (exact) {
         bx0 = rel_exact(pio.x, pf.x, C.hwid[0], C.dwid[0]);
@@ -669,13 +673,14 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
         bz0 = zi - pf.z;
     }
     double bx1 = bx0, by1 = by0, bz1 = bz0;
-    for (int off = 32; off >= S; off >>= 1) {
+    for (int off = 1; off < M; off <<= 1) {
         bx0 = fmin(bx0, __shfl_xor(bx0, off)); bx1 = fmax(bx1, __shfl_xor(bx1, off));
         by0 = fmin(by0, __shfl_xor(by0, off)); by1 = fmax(by1, __shfl_xor(by1, off));
         bz0 = fmin(bz0, __shfl_xor(bz0, off)); bz1 = fmax(bz1, __shfl_xor(bz1, off));
     }
     const double band = 1.0e-8 * C.cut2;
     const double bound = C.cut2 + band;
+    const double cut_lo = C.cut2 - band;
     // the box test needs one minimum image for all members: |x_j - x_first| < W/2 on every
     // axis for candidates within the cut-off of a member (members share a cell narrower than
     // two cut-offs), which eight cut-offs per width guarantee; narrower boxes test everything
@@ -700,7 +705,6 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
         L.srank[lane] = v;
     }
     __builtin_amdgcn_wave_barrier();
-    int ncell = 0;
     if (lane < nc) {
         int at = 0;
         for (int q = 0; q < nc; ++q) {
@@ -711,8 +715,7 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
         const int lo = start[k0];
         L.clo[at] = lo;
         L.cmid[at] = start[k0 + 1];
-        ncell = start[k0 + 2] - lo;
-        L.srank[at] = ncell;          // the counts in ascending rank (srank is read above)
+        L.srank[at] = start[k0 + 2] - lo;      // the counts in ascending rank (srank is read above)
     }
     __builtin_amdgcn_wave_barrier();
     // exclusive prefix of the counts over the cells in ascending rank
@@ -743,7 +746,13 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
         fl = j < mid;
     };
     const unsigned long long below = (1ull << lane) - 1ull;
-    int cnt = 0, fl_in = 0, ulen = 0, ulen1 = 0;
+    int ulen = 0, ulen1 = 0;
+    // per member slot m (uniform): its count in 16-bit field m mod 4 of cnt_lo (m < 4) or
+    // cnt_hi, and in bit m of flbits whether it has a fluid partner; bit m of pres: slot m holds
+    // a member
+    unsigned long long cnt_lo = 0, cnt_hi = 0;
+    unsigned flbits = 0;
+    const unsigned pres = ((1u << info.y) - 1u) | (P == 2 ? ((1u << info1.y) - 1u) << K : 0u);
     // per member m of each cluster, the number of its entries in the cluster's union (8 bits
     // each; more than one only where a stencil repeats cells)
     unsigned long long own = 0, own1 = 0;
@@ -753,64 +762,94 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
     // periodic arithmetic per pair)
     auto walk = [&](auto ex_tag) {
         constexpr bool EX = decltype(ex_tag)::value;
-        unsigned wp = 0, rp = 0;        // ring write and read counts (uniform)
-        // rows x slots over ring entries rp .. rp + nb - 1, then the block's union appended
+        // ring write and read counts (uniform); blocks are read 64 at a time, so a block never
+        // wraps around the ring's end
+        unsigned wp = 0, rp = 0;
+        // ring entries rp .. rp + nb - 1 against every member, then the block's union appended
         auto test_block = [&](int nb) {
+            const int b0 = (int)(rp & 127u);
+            const bool ok = lane < nb;
+            // lanes past the block's end hold stale LDS: the plain path gives them a far
+            // position (no hit, no exact re-test, no load), the exact path masks them
+            v2d cxy = L.xy[b0 + lane];
+            double cz = L.z[b0 + lane];
+            if (!EX && !ok) {
+                cxy = v2d{kFar, kFar};
+                cz = kFar;
+            }
+            const int cjf = L.jf[b0 + lane];
+            const int cj = cjf & 0x7fffffff;
+            const unsigned long long okm = EX ? ballot64(ok) : ~0ull;
+            const unsigned long long fm = ballot64(cjf < 0);
             unsigned long long ub = 0, ub1 = 0;
-            const int nt = (nb + S - 1) / S;
-#pragma unroll 1
-            for (int t = 0; t < nt; ++t) {
-                const int e = t * S + slot;
-                const int s = (int)((rp + (unsigned)e) & 127u);
-                const v2d xy = L.xy[s];
-                const ListCandZ cz = L.zj[s];
+            // hits of member slot m on the block's entries, and those that are not the member
+            auto member_hits = [&](int m, unsigned long long& hit, unsigned long long& other) {
+                const double4 pm = L.mem[m];
+                const int im = (int)__double_as_longlong(pm.w);
                 double q2;
                 if constexpr (!EX) {
-                    const double dx = xy.x - xi, dy = xy.y - yi, dz = cz.z - zi;
+                    const double dx = cxy.x - pm.x, dy = cxy.y - pm.y, dz = cz - pm.z;
                     q2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                    // (slots past the block's last entry hold stale LDS: no load from them)
-                    const bool edge = e < nb && fabs(q2 - C.cut2) <= band;
-                    if (__ballot(edge) != 0ull && edge) {
-                        const double4 pj = pos[cz.j];
-                        const double ex = rel_exact(pj.x, pio.x, C.hwid[0], C.dwid[0]);
-                        const double ey = rel_exact(pj.y, pio.y, C.hwid[1], C.dwid[1]);
-                        const double ez = rel_exact(pj.z, pio.z, C.hwid[2], C.dwid[2]);
+                    // within 1e-8 of the cut-off: the reference's arithmetic on the original
+                    // positions
+                    if (q2 >= cut_lo && q2 <= bound) {
+                        const double4 pj = pos[cj], pi = pos[im];
+                        const double ex = rel_exact(pj.x, pi.x, C.hwid[0], C.dwid[0]);
+                        const double ey = rel_exact(pj.y, pi.y, C.hwid[1], C.dwid[1]);
+                        const double ez = rel_exact(pj.z, pi.z, C.hwid[2], C.dwid[2]);
                         q2 = ex * ex + ey * ey + ez * ez;
                     }
                 } else {
-                    const double dx = rel_exact(xy.x, pio.x, C.hwid[0], C.dwid[0]);
-                    const double dy = rel_exact(xy.y, pio.y, C.hwid[1], C.dwid[1]);
-                    const double dz = rel_exact(cz.z, pio.z, C.hwid[2], C.dwid[2]);
+                    const double dx = rel_exact(cxy.x, pm.x, C.hwid[0], C.dwid[0]);
+                    const double dy = rel_exact(cxy.y, pm.y, C.hwid[1], C.dwid[1]);
+                    const double dz = rel_exact(cz, pm.z, C.hwid[2], C.dwid[2]);
                     q2 = dx * dx + dy * dy + dz * dz;
                 }
-                const bool in = mine && e < nb && q2 <= C.cut2;
-                cnt += in;
-                fl_in |= in ? cz.fluid : 0;
-                const unsigned long long bv = __ballot(in && cz.j != i);
-                // OR of each cluster's K row chunks of S bits
-                constexpr unsigned long long smask = S == 64 ? ~0ull : ((1ull << S) - 1ull);
-                unsigned long long u = P == 2 ? (bv & 0xffffffffull) : bv, u1 = P == 2 ? (bv >> 32) : 0ull;
-                for (int sh = 32 / P; sh >= S; sh >>= 1) {
-                    u |= u >> sh;
-                    if (P == 2) u1 |= u1 >> sh;
+                hit = ballot64(q2 <= C.cut2) & okm;
+                other = hit & ~ballot64(cj == im);
+            };
+            // the plain path unrolls the members (static slots), the exact path does not
+            auto account = [&](int m, unsigned long long hit, unsigned long long other) {
+                const unsigned long long pc = (unsigned long long)__popcll(hit) << (16 * (m & 3));
+                if (m < 4)
+                    cnt_lo += pc;
+                else
+                    cnt_hi += pc;
+                if ((hit & fm) != 0ull) flbits |= 1u << m;
+                if (P == 2 && m >= K)
+                    ub1 |= other;
+                else
+                    ub |= other;
+            };
+            if constexpr (!EX) {
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    if (!((pres >> m) & 1u)) continue;      // absent member (uniform)
+                    unsigned long long hit, other;
+                    member_hits(m, hit, other);
+                    account(m, hit, other);
                 }
-                ub |= (u & smask) << (t * S);
-                if (P == 2) ub1 |= (u1 & smask) << (t * S);
+            } else {
+#pragma unroll 1
+                for (int m = 0; m < M; ++m) {
+                    if (!((pres >> m) & 1u)) continue;
+                    unsigned long long hit, other;
+                    member_hits(m, hit, other);
+                    account(m, hit, other);
+                }
             }
-            const int s = (int)((rp + (unsigned)lane) & 127u);
-            const int j = L.zj[s].j;
             if ((ub >> lane) & 1ull) {
                 const int at = ulen + (int)__popcll(ub & below);
-                if (at < width) out[at] = j;
-                const unsigned m = (unsigned)(j - info.x);
+                if (at < width) out[at] = cj;
+                const unsigned m = (unsigned)(cj - info.x);
                 if (m < (unsigned)info.y) own += 1ull << (8 * m);
             }
             ulen += (int)__popcll(ub);
             if (P == 2) {
                 if ((ub1 >> lane) & 1ull) {
                     const int at = ulen1 + (int)__popcll(ub1 & below);
-                    if (at < width) out1[at] = j;
-                    const unsigned m = (unsigned)(j - info1.x);
+                    if (at < width) out1[at] = cj;
+                    const unsigned m = (unsigned)(cj - info1.x);
                     if (m < (unsigned)info1.y) own1 += 1ull << (8 * m);
                 }
                 ulen1 += (int)__popcll(ub1);
@@ -821,7 +860,9 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
         q0 = 0;
         if (total > 0) locate(0, jn, fn);
         double4 pn = lane < total ? pos[jn] : make_double4(0.0, 0.0, 0.0, 0.0);
-        for (int base = 0; base < total; base += 64) {
+        // one loop for the blocks and the ring's last entries (one copy of the test code)
+        for (int base = 0;; base += 64) {
+          if (base < total) {
             // this block's candidates near the box, appended to the ring in candidate order
             double px = pn.x, py = pn.y, pz = pn.z;
             if (!EX && img) image_axes(C, ax, pf, px, py, pz);
@@ -842,15 +883,12 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
                 const double ez = fmax(fmax(dz - bz1, bz0 - dz), 0.0);
                 kp = fma(ez, ez, fma(ey, ey, ex * ex)) <= bound;
             }
-            const unsigned long long kb = __ballot(kp);
+            const unsigned long long kb = ballot64(kp);
             if (kp) {
-                const int s = (int)((wp + (unsigned)__popcll(kb & below)) & 127u);
-                L.xy[s] = v2d{px, py};
-                ListCandZ cz;
-                cz.z = pz;
-                cz.j = jn;
-                cz.fluid = fn;
-                L.zj[s] = cz;
+                const unsigned at = (wp + (unsigned)__popcll(kb & below)) & 127u;
+                L.xy[at] = v2d{px, py};
+                L.z[at] = pz;
+                L.jf[at] = fn ? (jn | (int)0x80000000u) : jn;
             }
             wp += (unsigned)__popcll(kb);
             // the next block's loads are in flight while the ring is tested
@@ -858,27 +896,26 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
                 locate(base + 64, jn, fn);
                 if (base + 64 + lane < total) pn = pos[jn];
             }
+          }
+            const bool last = base + 64 >= total;
             __builtin_amdgcn_wave_barrier();
-            if (wp - rp >= 64u) test_block(64);
+            if (wp - rp >= 64u || (last && wp != rp)) test_block(wp - rp >= 64u ? 64 : (int)(wp - rp));
             __builtin_amdgcn_wave_barrier();
+            if (last && wp == rp) break;
         }
-        if (wp != rp) test_block((int)(wp - rp));
     };
     if (exact)
         walk(std::true_type{});
     else
         walk(std::false_type{});
-    for (int off = S / 2; off > 0; off >>= 1) {
-        cnt += __shfl_xor(cnt, off, S);
-        fl_in |= __shfl_xor(fl_in, off, S);
-    }
     for (int off = 32; off > 0; off >>= 1) {
         own += __shfl_xor(own, off);
         if (P == 2) own1 += __shfl_xor(own1, off);
     }
     // NeighborCount: the member's own entry included; a wall particle with no fluid partner
     // has none (main.cpp:1416)
-    if (mine && slot == 0) count_out[i] = (wall_i && fl_in == 0) ? 0 : cnt;
+    const int cnt = (int)(((lane < 4 ? cnt_lo : cnt_hi) >> (16 * (lane & 3))) & 0xffffull);
+    if (lane < M && present) count_out[i] = (wall_i && ((flbits >> lane) & 1u) == 0u) ? 0 : cnt;
     if (lane < info.y) selfin[info.x + lane] = (int)((own >> (8 * lane)) & 0xffull);
     if (P == 2 && lane < info1.y) selfin[info1.x + lane] = (int)((own1 >> (8 * lane)) & 0xffull);
     if (lane == 0) cl_len[g] = ulen < width ? ulen : width;
