@@ -155,15 +155,50 @@ def batch_leg(args, Solver):
 SUR_SCALES = dict(u_scale=0.1, a_scale=10.0, dens_scale=50.0)
 
 
+def pretrain_decks():
+    """The other trajectories the network is trained on before the measured deck's own: the
+    2D square droplet of 80 x 80 and the 3D dam break of 64 x 32 x 64 (plain FP64 runs)."""
+    from mphexplicit_amd import decks
+    return [("droplet2d_80", decks.droplet_2d()), ("dambreak3d_64", decks.dam_break_3d())]
+
+
+def pretrain(args):
+    """Weights trained on each pre-training deck's plain trajectory in turn (from step
+    --sur-warm, --sur-samples samples x --sur-iters Adam steps each; the Adam moments start
+    afresh on each deck), as the measured deck's training then continues from them."""
+    from mphexplicit_amd.solver import Solver, Surrogate
+    w, info = None, []
+    for name, d in pretrain_decks():
+        t0 = time.perf_counter()
+        s = Solver.from_deck(d)
+        s.step(args.sur_warm)
+        sur = Surrogate(s, train_stride=args.sur_stride, lr=1e-3, seed=5, radius_ratio=args.sur_radius,
+                        **SUR_SCALES)
+        if w is not None:
+            sur.set_weights(w)
+        losses = sur.train(args.sur_samples, args.sur_iters)
+        w = sur.weights()
+        sur.close()
+        s.close()
+        info.append({"deck": name, "particles": d.n, "loss_first": float(losses[0]),
+                     "loss_last": float(losses[-1]), "s": time.perf_counter() - t0})
+        log("pre-training on %s: loss %.3g -> %.3g" % (name, losses[0], losses[-1]))
+    return w, info
+
+
 def surrogate_leg(args, deck):
     """Learned 2 dt steps on the same deck: train on the deck's own plain trajectory (from a
-    developed flow), then time plain steps, bare surrogate steps and judged blocks (K
-    surrogate steps, one-step judge against two plain steps, rollback above the bound) in the
-    same stretch of the flow."""
+    developed flow; with pre-training, continuing from the weights trained on the 2D droplet
+    and the 64 x 32 x 64 dam break), then time plain steps, bare surrogate steps and judged
+    blocks (K surrogate steps, one-step judge against two plain steps, rollback above the
+    bound) in the same stretch of the flow."""
     from mphexplicit_amd.solver import Solver, Surrogate
+    w0, pre = pretrain(args) if args.sur_pretrain else (None, [])
     s = Solver.from_deck(deck)
     s.step(args.sur_warm)
     sur = Surrogate(s, train_stride=args.sur_stride, lr=1e-3, seed=5, radius_ratio=args.sur_radius, **SUR_SCALES)
+    if w0 is not None:
+        sur.set_weights(w0)
     s.sync()
     t0 = time.perf_counter()
     losses = sur.train(args.sur_samples, args.sur_iters)
@@ -186,7 +221,7 @@ def surrogate_leg(args, deck):
            "bare_sim_seconds_per_s": 2.0 * dt / t_bare,
            "judged": st, "judged_sim_seconds_per_s": st["sim_time"] / t_judged,
            "plain_sim_seconds_per_s": plain_sim, "tol": args.sur_tol, "k": args.sur_k,
-           "radius_ratio": args.sur_radius or deck.data.radius_ratio_p}
+           "radius_ratio": args.sur_radius or deck.data.radius_ratio_p, "pretrained_on": pre}
     out["judged_vs_plain"] = out["judged_sim_seconds_per_s"] / plain_sim
     out["bare_vs_plain"] = out["bare_sim_seconds_per_s"] / plain_sim
     sur.close()
@@ -217,6 +252,9 @@ def main():
     ap.add_argument("--sur-blocks", type=int, default=8)
     ap.add_argument("--sur-k", type=int, default=10)
     ap.add_argument("--sur-tol", type=float, default=0.01)
+    ap.add_argument("--sur-pretrain", action="store_true",
+                    help="train on the 2D droplet (80x80) and 64x32x64 dam break trajectories first "
+                         "(measured worse on the 1M deck: more blocks rolled back)")
     ap.add_argument("--sur-radius", type=float, default=0.0,
                     help="message radius of the surrogate / l0 (0: the largest kernel radius)")
     ap.add_argument("--late-steps", type=int, default=200,

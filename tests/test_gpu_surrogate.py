@@ -122,6 +122,28 @@ def test_training_lowers_loss(mods):
     assert last < 0.3 * first
 
 
+def test_training_across_decks(mods):
+    # one network trained on trajectories of several decks in turn (a 2D droplet, then a 3D
+    # dam break), as the surrogate of the 1M-particle deck is: the weights carry from one
+    # context to the next, and on the dam break training goes on from them
+    w2d, l2d = _trained_weights(mods, decks.small_droplet_2d(), samples=20, iters=10, lr=2e-3)
+    s = mods.Solver.from_deck(decks.small_dam_break_3d())
+    s.step(6)
+    sur = mods.Surrogate(s, seed=9, lr=2e-3, **SCALES)
+    sur.set_weights(w2d)
+    assert np.array_equal(sur.weights(), w2d)
+    losses = sur.train(20, 10)
+    w = sur.weights()
+    print("2D droplet loss %.3e -> %.3e; dam break %.3e -> %.3e" % (l2d[0], l2d[-1], losses[0], losses[-1]),
+          flush=True)
+    assert np.all(np.isfinite(l2d)) and np.all(np.isfinite(losses))
+    assert l2d[-10:].mean() < l2d[:5].mean()
+    assert losses[-10:].mean() < losses[:5].mean()
+    assert not np.array_equal(w, w2d)
+    sur.close()
+    s.close()
+
+
 def _errors(a, b):
     fl = b["type"] < 2
     ev = np.linalg.norm(a["velocity"][fl] - b["velocity"][fl]) / np.linalg.norm(b["velocity"][fl])
