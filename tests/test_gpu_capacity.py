@@ -88,6 +88,31 @@ def test_paths_bitwise_after_growth(solver_cls, monkeypatch):
         assert out[mode][1]["list_width"] == out["eager"][1]["list_width"]
 
 
+def test_rejected_block_that_outgrew_the_lists(solver_cls, monkeypatch):
+    # a judged block whose surrogate steps outgrow the list width is rejected and replayed
+    # with plain steps: the call succeeds and leaves the plain path's results (the replay
+    # starts with a rebuild, so sums are ordered differently from an uninterrupted run)
+    monkeypatch.setenv("MPH_TUNE", "4")
+    from mphexplicit_amd.solver import Surrogate
+    d = decks.collapsing_cloud_3d()
+    a = solver_cls.from_deck(d)
+    b = solver_cls.from_deck(d)
+    w0 = b.stats()["list_width"]
+    sur = Surrogate(b, seed=7)
+    sur.set_weights(np.zeros_like(sur.weights()))
+    stats = sur.advance(1, k=STEPS // 2, tol=1e-6)
+    print("rejected block:", stats, "width %d -> %d" % (w0, b.stats()["list_width"]), flush=True)
+    assert stats["blocks"] == 1 and stats["rollbacks"] == 1
+    assert b.stats()["list_width"] > w0
+    a.step(STEPS)
+    assert b.time() == pytest.approx(a.time(), rel=1e-12)
+    fa, fb = a.fields(), b.fields()
+    for k in ("position", "velocity", "pressure_p"):
+        err = normwise(fb[k], fa[k])
+        print("replayed vs plain %s %.2e" % (k, err), flush=True)
+        assert err <= 1e-9, (k, err)
+
+
 def test_surrogate_steps_widen_lists(solver_cls, monkeypatch):
     # a network with zero weights predicts no force: its steps carry the cloud in ballistically,
     # faster than the pressure lets the plain path, and the lists outgrow the width inside the
