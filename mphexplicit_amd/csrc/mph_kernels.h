@@ -747,10 +747,10 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
     };
     const unsigned long long below = (1ull << lane) - 1ull;
     int ulen = 0, ulen1 = 0;
-    // per member slot m (uniform): its count in 16-bit field m mod 4 of cnt_lo (m < 4) or
-    // cnt_hi, and in bit m of flbits whether it has a fluid partner; bit m of pres: slot m holds
-    // a member
-    unsigned long long cnt_lo = 0, cnt_hi = 0;
+    // per member slot m (uniform): its count in 32-bit field m mod 2 of cnt01 (m < 2), cnt23,
+    // cnt45 or cnt67, and in bit m of flbits whether it has a fluid partner; bit m of pres:
+    // slot m holds a member
+    unsigned long long cnt01 = 0, cnt23 = 0, cnt45 = 0, cnt67 = 0;
     unsigned flbits = 0;
     const unsigned pres = ((1u << info.y) - 1u) | (P == 2 ? ((1u << info1.y) - 1u) << K : 0u);
     // per member m of each cluster, the number of its entries in the cluster's union (8 bits
@@ -810,11 +810,15 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
             };
             // the plain path unrolls the members (static slots), the exact path does not
             auto account = [&](int m, unsigned long long hit, unsigned long long other) {
-                const unsigned long long pc = (unsigned long long)__popcll(hit) << (16 * (m & 3));
-                if (m < 4)
-                    cnt_lo += pc;
+                const unsigned long long pc = (unsigned long long)__popcll(hit) << (32 * (m & 1));
+                if (m < 2)
+                    cnt01 += pc;
+                else if (m < 4)
+                    cnt23 += pc;
+                else if (m < 6)
+                    cnt45 += pc;
                 else
-                    cnt_hi += pc;
+                    cnt67 += pc;
                 if ((hit & fm) != 0ull) flbits |= 1u << m;
                 if (P == 2 && m >= K)
                     ub1 |= other;
@@ -914,7 +918,8 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
     }
     // NeighborCount: the member's own entry included; a wall particle with no fluid partner
     // has none (main.cpp:1416)
-    const int cnt = (int)(((lane < 4 ? cnt_lo : cnt_hi) >> (16 * (lane & 3))) & 0xffffull);
+    const unsigned long long cpair = lane < 2 ? cnt01 : (lane < 4 ? cnt23 : (lane < 6 ? cnt45 : cnt67));
+    const int cnt = (int)((cpair >> (32 * (lane & 1))) & 0xffffffffull);
     if (lane < M && present) count_out[i] = (wall_i && ((flbits >> lane) & 1u) == 0u) ? 0 : cnt;
     if (lane < info.y) selfin[info.x + lane] = (int)((own >> (8 * lane)) & 0xffull);
     if (P == 2 && lane < info1.y) selfin[info1.x + lane] = (int)((own1 >> (8 * lane)) & 0xffull);
