@@ -468,6 +468,17 @@ def periodic_flow_3d(n_side=20, l0=1.0e-3, drift=(0.02, 0.05, 0.0), seed=16, ste
                 np.zeros(len(x), np.int32), x, v, steps=steps)
 
 
+def walls_only_3d(l0=1.0e-3, steps=5, name=None) -> Deck:
+    """Wall particles only (types 2 and 3, static), no fluid: every wall particle has no fluid
+    partner, so none has a list (main.cpp:1416) and every sum is zero."""
+    x = _lattice((10, 10, 6), l0, (0.0, 0.0, 0.0))
+    t = np.where(x[:, 2] < 3 * l0, 2, 3).astype(np.int32)
+    K, dt = sound_speed_params(1000.0, 1.0, l0)
+    data = DataParams(dt=dt, end_time=dt * steps, bulk_modulus=(K,) * 4, gravity=(0.0, 0.0, -9.8))
+    return Deck(name or "walls_only3d", 3, data, l0, np.zeros(3), np.array([20.0, 20.0, 12.0]) * l0,
+                t, x, np.zeros_like(x), steps=steps)
+
+
 def narrow_periodic_3d(shape=(16, 5, 8), l0=1.0e-3, seed=20, steps=20, name=None) -> Deck:
     """Fluid filling a periodic box with axes narrower than three cells (here one cell on y and
     two on z): the reference's stencil then visits cells more than once (main.cpp:1365-1391),

@@ -130,3 +130,23 @@ def test_smallest_cases_match_oracle(mod, n):
         print(n, k, "%.2e" % err, flush=True)
         assert err <= FP64_TOL, k
     s.close()
+
+
+def test_walls_only_deck_matches_oracle(mod):
+    # no fluid at all: no wall particle has a list, every sum is zero and static walls stay
+    from mphexplicit_amd import decks
+    d = decks.walls_only_3d()
+    orc = Oracle(d, nthreads=1)
+    s = mod.Solver.from_deck(d)
+    orc.step(5)
+    s.step(5)
+    of = by_id(orc.fields())
+    gf = s.fields()
+    assert np.array_equal(gf["neighbor_count"], of["ncount"])
+    assert not gf["neighbor_count"].any()
+    assert np.array_equal(gf["position"], of["x"])
+    for k in ("velocity", "force", "density_a", "pressure_p"):
+        assert not np.any(gf[k]), k
+    ptr, ind = s.neighbors()
+    assert len(ind) == 0 and not np.any(ptr)
+    s.close()

@@ -67,7 +67,8 @@ def test_oracle_matches_golden(name):
                     reason="reference build oracle/_ref not present")
 @pytest.mark.parametrize("factory,steps", [(decks.small_droplet_2d, 40), (decks.sample_square_drop, 30),
                                            (decks.small_moving_wall_2d, 40), (decks.small_two_fluid_3d, 6),
-                                           (decks.small_rotating_wall_3d, 40), (decks.narrow_periodic_3d, 20)])
+                                           (decks.small_rotating_wall_3d, 40), (decks.narrow_periodic_3d, 20),
+                                           (decks.walls_only_3d, 6)])
 def test_oracle_bitwise_vs_reference_build(factory, steps):
     d = factory()
     o, r = Oracle(d, nthreads=4), Reference(d, nthreads=4)
