@@ -678,6 +678,12 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
         by0 = fmin(by0, __shfl_xor(by0, off)); by1 = fmax(by1, __shfl_xor(by1, off));
         bz0 = fmin(bz0, __shfl_xor(bz0, off)); bz1 = fmax(bz1, __shfl_xor(bz1, off));
     }
+    // the box as centre and half-extents (widened by 1e-12 of the extent and a few ulp of the
+    // coordinates, far inside the test's margin): |d - c| - h per axis
+    const double bcx = 0.5 * (bx0 + bx1), bcy = 0.5 * (by0 + by1), bcz = 0.5 * (bz0 + bz1);
+    const double slack = 1.0e-12 * (bx1 - bx0 + by1 - by0 + bz1 - bz0) + 1.0e-15 * (fabs(bx0) + fabs(bx1) +
+                         fabs(by0) + fabs(by1) + fabs(bz0) + fabs(bz1));
+    const double bhx = 0.5 * (bx1 - bx0) + slack, bhy = 0.5 * (by1 - by0) + slack, bhz = 0.5 * (bz1 - bz0) + slack;
     const double band = 1.0e-8 * C.cut2;
     const double bound = C.cut2 + band;
     const double cut_lo = C.cut2 - band;
@@ -882,9 +888,9 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
                     dy = py - pf.y;
                     dz = pz - pf.z;
                 }
-                const double ex = fmax(fmax(dx - bx1, bx0 - dx), 0.0);
-                const double ey = fmax(fmax(dy - by1, by0 - dy), 0.0);
-                const double ez = fmax(fmax(dz - bz1, bz0 - dz), 0.0);
+                const double ex = fmax(fabs(dx - bcx) - bhx, 0.0);
+                const double ey = fmax(fabs(dy - bcy) - bhy, 0.0);
+                const double ez = fmax(fabs(dz - bcz) - bhz, 0.0);
                 kp = fma(ez, ez, fma(ey, ey, ex * ex)) <= bound;
             }
             const unsigned long long kb = ballot64(kp);
