@@ -30,7 +30,7 @@ enum mph_status {
     MPH_ERR_IO = -2,        /* file could not be opened */
     MPH_ERR_FORMAT = -3,    /* malformed .grid file */
     MPH_ERR_DEVICE = -4,    /* HIP runtime error or no usable GPU */
-    MPH_ERR_CAPACITY = -5,  /* a neighbour list outgrew its reserved width */
+    MPH_ERR_CAPACITY = -5,  /* a neighbour list outgrew its reserved width and could not be widened */
     MPH_ERR_STATE = -6      /* call not valid in the context's current state */
 };
 
@@ -160,7 +160,9 @@ mph_ctx *mph_create_batch(int ncases, const mph_params *p, const int *counts, co
 /* `nsteps` iterations of the reference's loop body (main.cpp:386-431) for every case:
  * convection, walls, periodic wrap, Verlet check + rebuild, densities, pressures,
  * surface tension, viscosity, gravity, acceleration; Time += Dt (main.cpp:449).
- * Asynchronous on the context's stream except for a capacity check at the end. */
+ * Asynchronous on the context's stream except for the read of the list-overflow word at the
+ * end; a rebuild that outgrew the reserved list width is widened and finished inside the call
+ * (the reference sizes NeighborInd at every rebuild, main.cpp:1487-1495). */
 int mph_step(mph_ctx *ctx, int nsteps);
 
 /* Capture one step into a HIP graph and replay it for `nsteps` (same results as mph_step). */
