@@ -87,6 +87,8 @@ struct mph_ctx {
     int same_pv = 0;               // every case has equal pressure and viscosity radii
     int same_ap = 0;               // every case has equal attractive and pressure radii
     int ratio_one = 0;             // every case has all InteractionRatio == 1
+    int dens_stale = 0;            // DensityA was not summed by the last step (no surface tension)
+    int graph_stale = 0;           // ... nor by the captured step
     int K = 4;                 // cluster size (rows per wavefront)
     int width = 0;             // reserved entries per cluster list
     int nslots = 0;            // cluster slots over all cases (capacity)
@@ -402,6 +404,16 @@ static int derive_case(const mph_params& p, int begin, int end, int nfluid, int 
         }                                                                             \
     } while (0)
 
+// surface-tension-free instantiations only
+#define LAUNCH_S0__(kern, K, CB, CB2, ...)                                            \
+    do {                                                                              \
+        switch (K) {                                                                  \
+        case 2: hipLaunchKernelGGL((kern<0, 2, CB, CB2>), __VA_ARGS__); break;        \
+        case 4: hipLaunchKernelGGL((kern<0, 4, CB, CB2>), __VA_ARGS__); break;        \
+        default: hipLaunchKernelGGL((kern<0, 8, CB, CB2>), __VA_ARGS__); break;       \
+        }                                                                             \
+    } while (0)
+
 #define LAUNCH_SK_(kern, SURF, K, CB, ...)                                            \
     do {                                                                              \
         if (SURF) {                                                                   \
@@ -517,11 +529,15 @@ static int launch_rebuild(mph_ctx* c, int count_only)
 // real: a step (advances wall centres); the autotuner launches with 0 so that repeated
 // launches leave the state unchanged.  count: count the interacting pairs (timed steps).
 // speculative: queued before the host knows whether the lists are rebuilt (returns at once
-// on the device if they are)
-static int launch_sweep_a(mph_ctx* c, int real, bool count = false, bool speculative = false)
+// on the device if they are).  need_na: sum DensityA also where nothing in the step needs it
+// (without surface tension the FP64 step skips it and marks it stale; ensure_dens_a sums it
+// when the fields are read)
+static int launch_sweep_a(mph_ctx* c, int real, bool count = false, bool speculative = false,
+                          bool need_na = false)
 {
     const int spec = speculative ? 1 : 0;
     if (c->mixed) {
+        c->dens_stale = 0;
         LAUNCH_SK(k_sweep_a_mixed, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream,
                   c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width,
                   c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p,
@@ -539,15 +555,39 @@ static int launch_sweep_a(mph_ctx* c, int real, bool count = false, bool specula
         if (cnt) LAUNCH_SK__(k_sweep_a, c->any_surface, c->K, true, M, SWEEP_A_ARGS);              \
         else LAUNCH_SK__(k_sweep_a, c->any_surface, c->K, false, M, SWEEP_A_ARGS);                 \
     } while (0)
-    switch (mode) {
-    case 0: SWEEP_A_MODE(0); break;
-    case 1: SWEEP_A_MODE(1); break;
-    case 2: SWEEP_A_MODE(2); break;
-    default: SWEEP_A_MODE(3); break;
+#define SWEEP_A_MODE0(M)                                                                           \
+    do {                                                                                           \
+        if (cnt) LAUNCH_S0__(k_sweep_a, c->K, true, M, SWEEP_A_ARGS);                              \
+        else LAUNCH_S0__(k_sweep_a, c->K, false, M, SWEEP_A_ARGS);                                 \
+    } while (0)
+    if (!c->any_surface && !need_na) {
+        switch (mode) {
+        case 0: SWEEP_A_MODE0(4); break;
+        case 1: SWEEP_A_MODE0(5); break;
+        case 2: SWEEP_A_MODE0(6); break;
+        default: SWEEP_A_MODE0(7); break;
+        }
+        c->dens_stale = 1;
+    } else {
+        switch (mode) {
+        case 0: SWEEP_A_MODE(0); break;
+        case 1: SWEEP_A_MODE(1); break;
+        case 2: SWEEP_A_MODE(2); break;
+        default: SWEEP_A_MODE(3); break;
+        }
+        c->dens_stale = 0;
     }
+#undef SWEEP_A_MODE0
 #undef SWEEP_A_MODE
 #undef SWEEP_A_ARGS
     return launch_ok("sweep A");
+}
+
+// DensityA of the current positions and lists, where the last step did not sum it: sweep A
+// again with n^A (its other outputs are rewritten with the same values)
+static int ensure_dens_a(mph_ctx* c)
+{
+    return c->dens_stale ? launch_sweep_a(c, 0, false, false, true) : MPH_OK;
 }
 
 static int launch_sweep_b(mph_ctx* c, int real)
@@ -1077,6 +1117,7 @@ extern "C" int mph_step_graph(mph_ctx* c, int nsteps)
         if (!c->graph) {
             HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             int rc = launch_step(c);
+            c->graph_stale = c->dens_stale;      // the captured step sums DensityA or not
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(c->stream, &g);
             if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -1088,6 +1129,7 @@ extern "C" int mph_step_graph(mph_ctx* c, int nsteps)
         int rc = steps_done(c, &done0), ran = 0;
         if (rc) return rc;
         for (int k = 0; k < left; ++k) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+        c->dens_stale = c->graph_stale;
         if ((rc = resolve_halt(c, done0, left, &ran))) return rc;
         advance_clock(c, ran);
         left -= ran;
@@ -1191,7 +1233,8 @@ extern "C" int mph_get_fields(mph_ctx* c, int case_id, int order, mph_fields* ou
     std::vector<double4> p4, v4, vn4, a4, f4;
     std::vector<int> idx, nc;
     std::vector<double> da, vs, dv, vp;
-    int rc = 0;
+    int rc = ensure_dens_a(c);
+    if (rc) return rc;
     if ((rc = fetch(c, c->pos.p, b, m, p4)) || (rc = fetch(c, c->vel.p, b, m, v4)) ||
         (rc = fetch(c, c->vnext.p, b, m, vn4)) || (rc = fetch(c, c->aux.p, b, m, a4)) ||
         (rc = fetch(c, c->force.p, b, m, f4)) || (rc = fetch(c, c->pidx.p, b, m, idx)) ||
@@ -1928,10 +1971,13 @@ extern "C" int mph_sur_advance(mph_surrogate* s, int blocks, int k, double tol, 
         for (int q = 0; q < k; ++q)
             if ((rc = sur_step(s))) return rc;
         // judge: one surrogate step and two plain steps from the same state
-        if ((rc = sur_ckpt_save(s, s->ck[1])) || (rc = sur_step(s)) || (rc = launch_sweep_a(c, 0))) return rc;
+        if ((rc = sur_ckpt_save(s, s->ck[1])) || (rc = sur_step(s)) || (rc = launch_sweep_a(c, 0, false, false, true)))
+            return rc;
         hipLaunchKernelGGL(k_sur_to_orig, dim3((c->n + 255) / 256), dim3(256), 0, st, c->n, c->pidx.p, c->vnext.p,
                            c->dens_a.p, s->v_ref.p, s->n_ref.p);
-        if ((rc = launch_ok("judge")) || (rc = sur_ckpt_load(s, s->ck[1])) || (rc = plain_steps(c, 2))) return rc;
+        if ((rc = launch_ok("judge")) || (rc = sur_ckpt_load(s, s->ck[1])) || (rc = plain_steps(c, 2)) ||
+            (rc = ensure_dens_a(c)))
+            return rc;
         HIP_TRY(hipMemsetAsync(s->sums.p, 0, 4 * sizeof(double), st));
         hipLaunchKernelGGL(k_sur_compare, dim3((c->n + 255) / 256), dim3(256), 0, st, c->n, c->pos.p, c->pidx.p,
                            c->vnext.p, c->dens_a.p, s->v_ref.p, s->n_ref.p, s->sums.p);

@@ -1238,6 +1238,7 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
 {
     constexpr bool SAMEAP = (MODE & 1) != 0;
     constexpr bool R1 = (MODE & 2) != 0 && !SURF;     // every InteractionRatio is 1: no tag needed
+    constexpr bool NA = (MODE & 4) == 0 || SURF;      // the number density n^A is summed
     auto fetch = [&](int j, Rec& r) {
         if (R1) {
             // no tag needed: x, y, z only (two fewer registers per record in flight)
@@ -1263,13 +1264,15 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         const double ir = rsqrt_pair3(q2);
         const double r = q2 * ir;
         const double oma = fmax(fma(-r, C.inv_ha, 1.0), 0.0);
-        const double ratio = (R1 || C.ratio_one) ? 1.0 : tab[16 + ti * 4 + (tag_of(q.p.w) & 3)];
-        const double wa = ratio * (oma * oma);
-        acc.na = fma(wa, r, acc.na);          // sum of w_A r; 1/h_A joins the prefactor
-        if (SURF) {
-            acc.gx = fma(dx, wa, acc.gx);
-            acc.gy = fma(dy, wa, acc.gy);
-            acc.gz = fma(dz, wa, acc.gz);
+        if (NA || SURF) {
+            const double ratio = (R1 || C.ratio_one) ? 1.0 : tab[16 + ti * 4 + (tag_of(q.p.w) & 3)];
+            const double wa = ratio * (oma * oma);
+            acc.na = fma(wa, r, acc.na);      // sum of w_A r; 1/h_A joins the prefactor
+            if (SURF) {
+                acc.gx = fma(dx, wa, acc.gx);
+                acc.gy = fma(dy, wa, acc.gy);
+                acc.gz = fma(dz, wa, acc.gz);
+            }
         }
         const double omp = SAMEAP ? oma : fmax(fma(-r, C.inv_hp, 1.0), 0.0);
         acc.ws = fma(omp, omp, acc.ws);
@@ -1286,7 +1289,11 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
 // COUNT: also count the interacting pairs (r <= largest radius) into pair_count; only the
 // timed steps that report pair rates use it.  MODE bit 0: the attractive and pressure
 // kernels have the same radius (one weight per pair); bit 1: every InteractionRatio is 1, so
-// without surface tension the partner records need no tag (three LDS planes)
+// without surface tension the partner records need no tag (three LDS planes); bit 2 (no
+// surface tension only): n^A is not summed and DensityA not written.  Without surface
+// tension the attractive pressure is zero (its coefficient a_t is), so n^A changes nothing
+// in the step and is an output field only; the host computes it when the fields are read,
+// by this kernel without bit 2 on the same positions and lists (the same sums, bit for bit).
 template <int SURF, int K, bool COUNT, int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 5 : 6, 8)))
 k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
@@ -1334,7 +1341,8 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     } else {
         sweep_a_pairs<SURF, K, 2, COUNT, MODE>(C, tab, f, ti, pi, vix, viy, viz, pos, vel, lds[w], acc, pi, 0);
     }
-    acc.na = slot_sum<K>(acc.na);
+    constexpr bool NA = (MODE & 4) == 0 || SURF;
+    if (NA) acc.na = slot_sum<K>(acc.na);
     acc.ws = slot_sum<K>(acc.ws);
     acc.ds = slot_sum<K>(acc.ds);
     if (SURF) {
@@ -1355,12 +1363,15 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
         const double dv = -C.dcp * acc.ds;
         const double kap = ep < 0.0 ? 0.0 : C.kbulk[ti];
         const double p = -C.lbulk[ti] * dv + kap * ep;
-        double pa = C.cofa[ti] * (na - C.n0a) / C.l0;
-        if (C.n0a <= na) pa = 0.0;
+        double pa = 0.0;
+        if (NA) {
+            pa = C.cofa[ti] * (na - C.n0a) / C.l0;
+            if (C.n0a <= na) pa = 0.0;
+        }
         const double gs = C.cg * C.g_scale;
         vel[i].w = p;
         aux[i] = make_double4(acc.gx * gs, acc.gy * gs, acc.gz * gs, pa);
-        dens_a[i] = na;
+        if (NA) dens_a[i] = na;
         vstrain[i] = ep;
         divp[i] = dv;
     } else {
