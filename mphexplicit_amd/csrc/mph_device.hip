@@ -537,11 +537,21 @@ static int launch_sweep_a(mph_ctx* c, int real, bool count = false, bool specula
 {
     const int spec = speculative ? 1 : 0;
     if (c->mixed) {
-        c->dens_stale = 0;
-        LAUNCH_SK(k_sweep_a_mixed, c->any_surface, c->K, dim3(c->ncblocks()), dim3(256), 0, c->stream,
-                  c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p, c->cl_nbr.p, c->cl_len.p, c->width,
-                  c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p, c->vstrain.p, c->divp.p,
-                  real ? c->pairs.p : c->pairs_scratch.p, c->selfin.p, c->any_flag.p, spec);
+#define SWEEP_AM_ARGS                                                                                       \
+    dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p,  \
+        c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p,          \
+        c->vstrain.p, c->divp.p, real ? c->pairs.p : c->pairs_scratch.p, c->selfin.p, c->any_flag.p, spec
+        const bool cntm = count && real;
+        if (!c->any_surface && !need_na) {
+            if (cntm) LAUNCH_S0__(k_sweep_a_mixed, c->K, true, false, SWEEP_AM_ARGS);
+            else LAUNCH_S0__(k_sweep_a_mixed, c->K, false, false, SWEEP_AM_ARGS);
+            c->dens_stale = 1;
+        } else {
+            if (cntm) LAUNCH_SK__(k_sweep_a_mixed, c->any_surface, c->K, true, true, SWEEP_AM_ARGS);
+            else LAUNCH_SK__(k_sweep_a_mixed, c->any_surface, c->K, false, true, SWEEP_AM_ARGS);
+            c->dens_stale = 0;
+        }
+#undef SWEEP_AM_ARGS
         return launch_ok("sweep A");
     }
     const int mode = (c->same_ap ? 1 : 0) | (c->ratio_one ? 2 : 0);

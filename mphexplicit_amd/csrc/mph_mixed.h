@@ -98,7 +98,7 @@ __device__ __forceinline__ void walk_union_lds_f(const Frame<K>& f, v4f* __restr
 // sweep A, mixed: same sums and epilogue as k_sweep_a (main.cpp:1665-1727, 1831-1906,
 // 1729-1740)
 
-template <int SURF, int K, int WRAP>
+template <int SURF, int K, int WRAP, bool COUNT, bool NA>
 __device__ __forceinline__ void sweep_a_pairs_f(const CaseConst& C, const float* tabf, const Frame<K>& f, int ti,
                                                 const double4& pf, float ox, float oy, float oz,
                                                 float vix, float viy, float viz,
@@ -133,18 +133,20 @@ __device__ __forceinline__ void sweep_a_pairs_f(const CaseConst& C, const float*
         // a row's own entry (zero offset difference) is kept off q^2 = 0 as in k_sweep_a; its
         // w_P(0) = 1 and pair count are taken back in the epilogue (selfin)
         const float q2 = fmaxf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)), 1.0e-36f);
-        acc.np += q2 <= hmax2;
+        if (COUNT) acc.np += q2 <= hmax2;
         const float ir = __builtin_amdgcn_rsqf(q2);
         const float r = q2 * ir;
-        const float sa = r * inv_ha;
-        const float oma = fmaxf(1.f - sa, 0.f);
-        const float ratio = C.ratio_one ? 1.f : tabf[16 + ti * 4 + __float_as_int(q.p.w)];
-        const float wa = ratio * (oma * oma);
-        acc.na += (double)(wa * sa);
-        if (SURF) {
-            gxf = fmaf(dx, wa, gxf);
-            gyf = fmaf(dy, wa, gyf);
-            gzf = fmaf(dz, wa, gzf);
+        if (NA || SURF) {
+            const float sa = r * inv_ha;
+            const float oma = fmaxf(1.f - sa, 0.f);
+            const float ratio = C.ratio_one ? 1.f : tabf[16 + ti * 4 + __float_as_int(q.p.w)];
+            const float wa = ratio * (oma * oma);
+            acc.na += (double)(wa * sa);
+            if (SURF) {
+                gxf = fmaf(dx, wa, gxf);
+                gyf = fmaf(dy, wa, gyf);
+                gzf = fmaf(dz, wa, gzf);
+            }
         }
         const float omp = fmaxf(fmaf(-r, inv_hp, 1.f), 0.f);
         acc.ws += (double)(omp * omp);
@@ -156,8 +158,9 @@ __device__ __forceinline__ void sweep_a_pairs_f(const CaseConst& C, const float*
 
 // Without surface tension the mixed sweeps are held to seven wavefronts per SIMD (72
 // VGPRs, a few bytes spilled outside the pair loop): measured faster than six and five and
-// than the compiler's choice (four).
-template <int SURF, int K>
+// than the compiler's choice (four).  COUNT: count the interacting pairs (timed steps); NA:
+// sum n^A (without surface tension only an output field, as in k_sweep_a).
+template <int SURF, int K, bool COUNT, bool NA>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 1 : 7, 8)))
 k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
                 const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
@@ -192,7 +195,7 @@ k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
     int axes;
     const int path = wrap_path(C, pi, axes);
 #define MPH_SWEEP_A_F(P)                                                                              \
-    sweep_a_pairs_f<SURF, K, P>(C, tabf, f, ti, pf, offset_f<P>(pi.x, pf.x, C.hwid[0], C.dwid[0], axes & 1), \
+    sweep_a_pairs_f<SURF, K, P, COUNT, NA || SURF>(C, tabf, f, ti, pf, offset_f<P>(pi.x, pf.x, C.hwid[0], C.dwid[0], axes & 1), \
                                 offset_f<P>(pi.y, pf.y, C.hwid[1], C.dwid[1], axes & 2),              \
                                 offset_f<P>(pi.z, pf.z, C.hwid[2], C.dwid[2], axes & 4), vix, viy, viz, \
                                 pos, vel, lds[w], acc, gxf, gyf, gzf, axes)
@@ -203,7 +206,7 @@ k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
     else
         MPH_SWEEP_A_F(2);
 #undef MPH_SWEEP_A_F
-    acc.na = slot_sum<K>(acc.na);
+    if (NA || SURF) acc.na = slot_sum<K>(acc.na);
     acc.ws = slot_sum<K>(acc.ws);
     acc.ds = slot_sum<K>(acc.ds);
     if (SURF) {
@@ -211,7 +214,7 @@ k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
         acc.gy = slot_sum<K>((double)gyf);
         acc.gz = slot_sum<K>((double)gzf);
     }
-    int npair = slot_sum_i<K>(acc.np);
+    int npair = COUNT ? slot_sum_i<K>(acc.np) : 0;
     if (f.slot == 0 && f.valid) {
         const int i = f.i;
         const int own = selfin[i];
@@ -224,21 +227,26 @@ k_sweep_a_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
         const double dv = -C.dcp * acc.ds;
         const double kap = ep < 0.0 ? 0.0 : C.kbulk[ti];
         const double p = -C.lbulk[ti] * dv + kap * ep;
-        double pa = C.cofa[ti] * (na - C.n0a) / C.l0;
-        if (C.n0a <= na) pa = 0.0;
+        double pa = 0.0;
+        if (NA || SURF) {
+            pa = C.cofa[ti] * (na - C.n0a) / C.l0;
+            if (C.n0a <= na) pa = 0.0;
+        }
         const double gs = C.cg * C.g_scale;
         vel[i].w = p;
         aux[i] = make_double4(acc.gx * gs, acc.gy * gs, acc.gz * gs, pa);
-        dens_a[i] = na;
+        if (NA || SURF) dens_a[i] = na;
         vstrain[i] = ep;
         divp[i] = dv;
     } else {
         npair = 0;
     }
-    for (int off = 32; off > 0; off >>= 1) npair += __shfl_xor(npair, off);
-    if ((threadIdx.x & 63) == 0 && npair) {
-        const int slot = (int)((blockIdx.x * 4u + (threadIdx.x >> 6)) & (kPairSlots - 1));
-        atomicAdd(pair_count + slot * kPairStride, (unsigned long long)npair);
+    if (COUNT) {
+        for (int off = 32; off > 0; off >>= 1) npair += __shfl_xor(npair, off);
+        if ((threadIdx.x & 63) == 0 && npair) {
+            const int slot = (int)((blockIdx.x * 4u + (threadIdx.x >> 6)) & (kPairSlots - 1));
+            atomicAdd(pair_count + slot * kPairStride, (unsigned long long)npair);
+        }
     }
 }
 
