@@ -353,7 +353,7 @@ def main():
         "config": {"workload": deck.name, "particles": deck.n, "fluid": int((deck.type < 2).sum()),
                    "precision": args.precision, "dt": deck.data.dt, "graph": bool(args.graph),
                    "list_width": st["list_width"],
-                   "cluster_size": st["cluster_size"]},
+                   "cluster_size": st["cluster_size"], "list_grid": st["list_grid"]},
         "timed_window": {"from_step": win["from_step"], "steps": args.steps, "rebuilds": win["rebuilds"],
                          "note": "value: steps/s of the --steps steps from step --from-step (developed flow)"},
         "start_of_flow": win["start_of_flow"],

@@ -101,6 +101,8 @@ typedef struct mph_stats {
     long long eval_slots;             /* (member, list entry) evaluations of one pair sweep over
                                          the current lists, padding and non-interacting entries
                                          included */
+    int list_grid;                    /* largest grid of the list build (workgroups), picked by
+                                         the autotuner for a single case */
 } mph_stats;
 
 /* Per-phase device time of mph_step_timed, summed over its steps. */

@@ -66,6 +66,7 @@ class MphStats(C.Structure):
         ("cluster_size", C.c_int),
         ("clusters", C.c_int),
         ("eval_slots", C.c_longlong),
+        ("list_grid", C.c_int),
     ]
 
 

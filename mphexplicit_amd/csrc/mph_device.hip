@@ -90,6 +90,7 @@ struct mph_ctx {
     int dens_stale = 0;            // DensityA was not summed by the last step (no surface tension)
     int graph_stale = 0;           // ... nor by the captured step
     int K = 4;                 // cluster size (rows per wavefront)
+    int lcap = 0;              // largest grid of the list build (workgroups)
     int width = 0;             // reserved entries per cluster list
     int nslots = 0;            // cluster slots over all cases (capacity)
     long long steps = 0;
@@ -471,6 +472,13 @@ static void launch_scan(mph_ctx* c, unsigned* data, int total)
                        c->any_flag.p);
 }
 
+// workgroups of the list build: the autotuner's choice (c->lcap), else the default cap
+static int list_grid(const mph_ctx* c)
+{
+    const int lcap = c->lcap > 0 ? c->lcap : (c->ncases == 1 ? 20480 : 4096);
+    return std::max(1, std::min((c->nslots + 3) / 4, lcap));
+}
+
 // the union lists of the cases flagged for a rebuild (count_only: only their lengths)
 static int launch_lists(mph_ctx* c, int count_only)
 {
@@ -479,9 +487,9 @@ static int launch_lists(mph_ctx* c, int count_only)
     // 4096 blocks, 2.30 at 10240, 2.27 at 20480 = 16 rounds at 5 wavefronts per SIMD).  In a
     // batch a rebuild usually covers a few cases, and the blocks that only skip slots cost
     // more than the balance gains (64-case batch: rebuild phase 0.60 ms at 4096, 0.73 at
-    // 20480).  A step without a rebuild exits in about 5 microseconds either way.
-    const int lcap = c->ncases == 1 ? 20480 : 4096;
-    const int lgrid = std::max(1, std::min((c->nslots + 3) / 4, lcap));
+    // 20480).  A step without a rebuild exits in about 5 microseconds either way.  For a
+    // single case the autotuner times the grids of kListGrids and keeps the fastest.
+    const int lgrid = list_grid(c);
     LAUNCH_K(k_cluster_lists, c->K, dim3((unsigned)lgrid), dim3(256), 0, c->stream, c->d_cases.p,
              c->slot_case.p, c->nslots, c->flags.p, c->any_flag.p, c->pos.p, c->start.p, c->rank.p,
              c->cl_info.p, c->cl_count.p, count_only ? 0 : c->width, c->cl_nbr.p, c->cl_len.p, c->selfin.p, c->nout.p,
@@ -816,10 +824,11 @@ static int prepare_clusters(mph_ctx* c, int K)
 
 // ---------------------------------------------------------------------------------------
 // autotuner: time the sweeps for every cluster size on the context's own particles and
-// keep the fastest.  Results are cached in a text file (MPH_TUNE_CACHE, default
-// ~/.cache/mphexplicit_amd/cluster_size.txt) keyed by device, dimension, surface tension,
-// precision, case count and problem size; MPH_TUNE=0 keeps K = 4, MPH_TUNE=<2|4|8> forces a size and
-// MPH_TUNE=force ignores the cache.
+// keep the fastest; for a single case, then time full list builds at several grid sizes
+// and keep the fastest.  Results are cached in a text file (MPH_TUNE_CACHE, default
+// ~/.cache/mphexplicit_amd/tiles.txt, lines "key K grid") keyed by device, dimension, surface
+// tension, precision, case count and problem size; MPH_TUNE=0 keeps K = 4, MPH_TUNE=<2|4|8>
+// forces a size (the grid keeps its default) and MPH_TUNE=force ignores the cache.
 
 static std::string tune_key(mph_ctx* c)
 {
@@ -843,7 +852,7 @@ static std::string tune_path()
     const char* p = std::getenv("MPH_TUNE_CACHE");
     if (p && *p) return p;
     const char* h = std::getenv("HOME");
-    return std::string(h ? h : "/tmp") + "/.cache/mphexplicit_amd/cluster_size.txt";
+    return std::string(h ? h : "/tmp") + "/.cache/mphexplicit_amd/tiles.txt";
 }
 
 static int time_sweeps(mph_ctx* c, float* ms_out)
@@ -869,6 +878,35 @@ static int time_sweeps(mph_ctx* c, float* ms_out)
     return rc;
 }
 
+// a full build of every list, timed (the cases are flagged for a rebuild, so each launch
+// rebuilds the same lists)
+static int time_lists(mph_ctx* c, float* ms_out)
+{
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = MPH_OK;
+    float total = 0.0f;
+    for (int rep = 0; rep < 4 && rc == MPH_OK; ++rep) {
+        (void)hipEventRecord(e0, c->stream);
+        rc = launch_lists(c, 0);
+        (void)hipEventRecord(e1, c->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0) total += ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_out = total;
+    return rc;
+}
+
+// list-build grids the tuner tries for a single case (4 wavefronts per workgroup looping
+// over the cluster slots); batches keep 4096 workgroups, where a rebuild usually covers a
+// few cases and blocks that only skip slots cost more than the balance gains
+constexpr int kListGrids[] = {4096, 10240, 20480, 40960};
+
 static int choose_cluster_size(mph_ctx* c, int* K)
 {
     const char* mode = std::getenv("MPH_TUNE");
@@ -882,10 +920,11 @@ static int choose_cluster_size(mph_ctx* c, int* K)
     if (!(mode && !std::strcmp(mode, "force"))) {
         if (FILE* f = std::fopen(path.c_str(), "r")) {
             char k[256];
-            int v;
-            while (std::fscanf(f, "%255s %d", k, &v) == 2)
-                if (key == k && (v == 2 || v == 4 || v == 8)) {
+            int v, g;
+            while (std::fscanf(f, "%255s %d %d", k, &v, &g) == 3)
+                if (key == k && (v == 2 || v == 4 || v == 8) && g >= 0) {
                     *K = v;
+                    c->lcap = g;
                     std::fclose(f);
                     return MPH_OK;
                 }
@@ -904,6 +943,19 @@ static int choose_cluster_size(mph_ctx* c, int* K)
     }
     c->tune_v.release();
     c->tune_f.release();
+    // the list build's grid for the chosen size, timed on full rebuilds of this case
+    if (!rc && c->ncases == 1) {
+        rc = prepare_clusters(c, *K);
+        float bl = 1e30f;
+        int bg = 0;
+        for (int q = 0; q < (int)(sizeof(kListGrids) / sizeof(kListGrids[0])) && rc == MPH_OK; ++q) {
+            c->lcap = kListGrids[q];
+            float ms = 0.0f;
+            rc = time_lists(c, &ms);
+            if (!rc && ms < bl) { bl = ms; bg = kListGrids[q]; }
+        }
+        c->lcap = bg;
+    }
     if (rc) return rc;
     // the cache keeps later contexts of the same shape on the same size (sizes round
     // differently); failing to write it is reported, and this context keeps its choice
@@ -912,10 +964,10 @@ static int choose_cluster_size(mph_ctx* c, int* K)
     if (cut != std::string::npos && cut > 0) std::filesystem::create_directories(path.substr(0, cut), ec);
     FILE* f = ec ? nullptr : std::fopen(path.c_str(), "a");
     if (f) {
-        std::fprintf(f, "%s %d\n", key.c_str(), *K);
+        std::fprintf(f, "%s %d %d\n", key.c_str(), *K, c->lcap);
         std::fclose(f);
     } else {
-        std::fprintf(stderr, "mphexplicit_amd: cannot write the cluster-size cache %s (%s)\n", path.c_str(),
+        std::fprintf(stderr, "mphexplicit_amd: cannot write the tile cache %s (%s)\n", path.c_str(),
                      ec ? ec.message().c_str() : std::strerror(errno));
     }
     return MPH_OK;
@@ -1481,6 +1533,7 @@ extern "C" int mph_get_stats(const mph_ctx* cc, mph_stats* out)
     out->list_width = c->width;
     out->max_list_len = mx;
     out->cluster_size = c->K;
+    out->list_grid = list_grid(c);
     out->clusters = 0;
     for (int v : cnt) out->clusters += v;
     return MPH_OK;

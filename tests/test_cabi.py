@@ -32,7 +32,8 @@ def test_struct_sizes_match_header_layout():
     # mph_params: 2 ints + 63 doubles (no padding after the two ints on x86-64)
     assert C.sizeof(MphParams) == 8 + 8 * (1 + 3 + 3 + 5 + 3 + 5 * 4 + 16 + 3 + 12 * 3)
     assert C.sizeof(MphFields) == 13 * 8
-    assert C.sizeof(MphStats) == 3 * 8 + 4 * 4 + 8
+    # mph_stats: 3 long long, 4 int, eval_slots, list_grid and 4 bytes of tail padding
+    assert C.sizeof(MphStats) == 3 * 8 + 4 * 4 + 8 + 4 + 4
 
 
 def test_params_default_matches_reference_statics():
