@@ -640,6 +640,23 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
 {
     constexpr int M = K * P;                           // member slots
     const int lane = threadIdx.x & 63;
+    // stencil cells: lane q < nc takes the q-th cell of the window (its rank and particle
+    // range are loaded first, so that this chain of loads overlaps the members' loads and
+    // box below), then its place in ascending rank (repeated cells, on axes of fewer than 3
+    // cells, in lane order)
+    const int cid = info.z;
+    const int iz = cid % C.cc[2];
+    const int iy = (cid / C.cc[2]) % C.cc[1];
+    const int ix = cid / (C.cc[1] * C.cc[2]);
+    int wx[3], wy[3], wz[3];
+    const int ncx = axis_window(ix, C.cc[0], 1, wx);
+    const int ncy = axis_window(iy, C.cc[1], 1, wy);
+    const int ncz = axis_window(iz, C.cc[2], C.dim == 2 ? 0 : 1, wz);
+    const int nc = ncx * ncy * ncz;
+    const int qc = lane < nc ? lane : 0;
+    const int v = rank[C.rank_base + (wx[qc / (ncy * ncz)] * C.cc[1] + wy[(qc / ncz) % ncy]) * C.cc[2] + wz[qc % ncz]];
+    const int sk = C.key_base + 2 * v;
+    const int slo = start[sk], smid = start[sk + 1], shi = start[sk + 2];
     // lane l < M takes member slot l; the other lanes repeat the slots (l mod M), and slots
     // of absent members repeat the cluster's first member (which does not change the box)
     const int ms = lane % M;
@@ -693,23 +710,7 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
     const double w8 = 64.0 * C.cut2;
     const bool use_box = C.dwid[0] * C.dwid[0] >= w8 && C.dwid[1] * C.dwid[1] >= w8 &&
                          (C.dim == 2 || C.dwid[2] * C.dwid[2] >= w8);
-    // stencil cells: lane q < nc takes the q-th cell of the window, then its place in
-    // ascending rank (repeated cells, on axes of fewer than 3 cells, in lane order)
-    const int cid = info.z;
-    const int iz = cid % C.cc[2];
-    const int iy = (cid / C.cc[2]) % C.cc[1];
-    const int ix = cid / (C.cc[1] * C.cc[2]);
-    int wx[3], wy[3], wz[3];
-    const int ncx = axis_window(ix, C.cc[0], 1, wx);
-    const int ncy = axis_window(iy, C.cc[1], 1, wy);
-    const int ncz = axis_window(iz, C.cc[2], C.dim == 2 ? 0 : 1, wz);
-    const int nc = ncx * ncy * ncz;
-    int v = 0;
-    if (lane < nc) {
-        const int a = lane / (ncy * ncz), b = (lane / ncz) % ncy, d = lane % ncz;
-        v = rank[C.rank_base + (wx[a] * C.cc[1] + wy[b]) * C.cc[2] + wz[d]];
-        L.srank[lane] = v;
-    }
+    if (lane < nc) L.srank[lane] = v;
     __builtin_amdgcn_wave_barrier();
     if (lane < nc) {
         int at = 0;
@@ -717,11 +718,9 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
             const int u = L.srank[q];
             at += u < v || (u == v && q < lane);
         }
-        const int k0 = C.key_base + 2 * v;
-        const int lo = start[k0];
-        L.clo[at] = lo;
-        L.cmid[at] = start[k0 + 1];
-        L.srank[at] = start[k0 + 2] - lo;      // the counts in ascending rank (srank is read above)
+        L.clo[at] = slo;
+        L.cmid[at] = smid;
+        L.srank[at] = shi - slo;               // the counts in ascending rank (srank is read above)
     }
     __builtin_amdgcn_wave_barrier();
     // exclusive prefix of the counts over the cells in ascending rank
@@ -960,6 +959,8 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
         // slots in pairs (2t, 2t + 1): two clusters of one cell share a wavefront
         for (int t = blockIdx.x * 4 + wv; 2 * t < nslots; t += gridDim.x * 4) {
             const int g0 = 2 * t, g1 = g0 + 1;
+            // the cluster entries are loaded with the slots' cases, not after them
+            const int4 i0 = cl_info[g0], i1 = cl_info[g1 < nslots ? g1 : g0];
             const int c0 = __builtin_amdgcn_readfirstlane(slot_case[g0]);
             const bool v0 = flags[c0] != 0 && g0 - cases[c0].cl_base < cl_count[c0];      // wave-uniform
             int c1 = c0;
@@ -968,15 +969,12 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
                 c1 = __builtin_amdgcn_readfirstlane(slot_case[g1]);
                 v1 = flags[c1] != 0 && g1 - cases[c1].cl_base < cl_count[c1];
             }
-            if (v0 && v1 && c0 == c1) {
-                const int4 i0 = cl_info[g0], i1 = cl_info[g1];
-                if (i0.z == i1.z) {
-                    longest = max(longest, MPH_LIST_ONE(2, c0, g0, i0, g1, i1));
-                    continue;
-                }
+            if (v0 && v1 && c0 == c1 && i0.z == i1.z) {
+                longest = max(longest, MPH_LIST_ONE(2, c0, g0, i0, g1, i1));
+                continue;
             }
-            if (v0) longest = max(longest, MPH_LIST_ONE(1, c0, g0, cl_info[g0], 0, none));
-            if (v1) longest = max(longest, MPH_LIST_ONE(1, c1, g1, cl_info[g1], 0, none));
+            if (v0) longest = max(longest, MPH_LIST_ONE(1, c0, g0, i0, 0, none));
+            if (v1) longest = max(longest, MPH_LIST_ONE(1, c1, g1, i1, 0, none));
         }
     } else {
         for (int g = blockIdx.x * 4 + wv; g < nslots; g += gridDim.x * 4) {
