@@ -934,9 +934,11 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
     return ulen > ulen1 ? ulen : ulen1;    // the caller keeps the largest for the capacity check
 }
 
-// five wavefronts per SIMD (at most 96 VGPRs)
+// six wavefronts per SIMD (at most 80 VGPRs, 32 bytes per lane spilled outside the candidate
+// and ring loops): full rebuilds of the 1.27M-particle deck 1.21 -> 1.14 ms against five
+// (96 VGPRs, no spills); seven measured the same as six
 template <int K>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slot_case, int nslots,
                 const int* __restrict__ flags, int* __restrict__ any_flag,
                 const double4* __restrict__ pos, const int* __restrict__ start,
