@@ -104,6 +104,8 @@ struct mph_ctx {
     DevBuf<int> selfin;            // 1: the particle's own index is in its cluster's union list
     DevBuf<double> dens_a, vstrain, divp, wallc, vir;
     DevBuf<unsigned> keys_a, keys_b, hist, ccount, scan_sums;
+    DevBuf<unsigned> kfull;        // batches: sorted keys of all particles
+    DevBuf<int> pfull, coff, nsort;  // batches: sort sources, compacted case offsets, counts
     DevBuf<int> perm_a, perm_b, rank, cell_of_rank, cell_case, slot_case;
     DevBuf<int4> cl_info;
     DevBuf<int> cl_count, cl_nbr, cl_len;
@@ -466,10 +468,22 @@ static void launch_scan(mph_ctx* c, unsigned* data, int total)
     }
     const int tiles = (total + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_scan_tile_sums, dim3(tiles), dim3(kScanThreads), 0, s, data, total, c->scan_sums.p,
-                       c->any_flag.p);
+                       c->any_flag.p, nullptr);
     hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, c->scan_sums.p, tiles, c->any_flag.p);
     hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanThreads), 0, s, data, total, c->scan_sums.p,
-                       c->any_flag.p);
+                       c->any_flag.p, nullptr);
+}
+
+// the same for a length counted on the device (dtotal[0] <= max_total)
+static void launch_scan_dev(mph_ctx* c, unsigned* data, int max_total, const int* dtotal)
+{
+    hipStream_t s = c->stream;
+    const int tiles = std::max(1, (max_total + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(tiles), dim3(kScanThreads), 0, s, data, max_total, c->scan_sums.p,
+                       c->any_flag.p, dtotal);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, c->scan_sums.p, tiles, c->any_flag.p);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanThreads), 0, s, data, max_total, c->scan_sums.p,
+                       c->any_flag.p, dtotal);
 }
 
 // workgroups of the list build: the autotuner's choice (c->lcap), else the default cap
@@ -501,9 +515,21 @@ static int launch_rebuild(mph_ctx* c, int count_only)
 {
     const int nb = c->nblocks();
     hipStream_t s = c->stream;
-    unsigned* kf = c->sorted_keys();
-    hipLaunchKernelGGL(k_keys, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p, c->pos.p,
-                       c->flags.p, c->any_flag.p, c->rank.p, kf, c->keys_a.p, c->perm_a.p);
+    // a single case sorts all its particles in place (the keys of the last sort stay in the
+    // buffer the passes end in); a batch sorts only its flagged cases, compacted (nsort)
+    const bool batch = c->ncases > 1;
+    unsigned* kf = batch ? c->kfull.p : c->sorted_keys();
+    int* pf = batch ? c->pfull.p : c->sorted_perm();
+    const int* dn = batch ? c->nsort.p : nullptr;
+    if (batch) {
+        hipLaunchKernelGGL(k_sort_offsets, dim3(1), dim3(256), 0, s, c->d_cases.p, c->ncases, c->flags.p,
+                           c->any_flag.p, c->coff.p, c->nsort.p);
+        hipLaunchKernelGGL(k_keys_compact, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p, c->pos.p,
+                           c->flags.p, c->any_flag.p, c->rank.p, c->coff.p, c->keys_a.p, c->perm_a.p);
+    } else {
+        hipLaunchKernelGGL(k_keys, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p, c->pos.p,
+                           c->flags.p, c->any_flag.p, c->rank.p, kf, c->keys_a.p, c->perm_a.p);
+    }
     for (int ps = 0; ps < c->sort_passes; ++ps) {
         unsigned* kin = (ps & 1) ? c->keys_b.p : c->keys_a.p;
         unsigned* kout = (ps & 1) ? c->keys_a.p : c->keys_b.p;
@@ -511,15 +537,21 @@ static int launch_rebuild(mph_ctx* c, int count_only)
         int* vout = (ps & 1) ? c->perm_a.p : c->perm_b.p;
         const int shift = ps * kRadixBits;
         hipLaunchKernelGGL(k_radix_hist, dim3(c->sort_blocks), dim3(kSortThreads), 0, s, kin, c->n,
-                           shift, c->hist.p, c->any_flag.p);
-        launch_scan(c, c->hist.p, kRadix * c->sort_blocks);
+                           shift, c->hist.p, c->any_flag.p, dn);
+        if (batch)
+            launch_scan_dev(c, c->hist.p, kRadix * c->sort_blocks, c->nsort.p + 1);
+        else
+            launch_scan(c, c->hist.p, kRadix * c->sort_blocks);
         hipLaunchKernelGGL(k_radix_scatter, dim3(c->sort_blocks), dim3(kSortThreads), 0, s, kin, vin,
-                           kout, vout, c->n, shift, c->hist.p, c->any_flag.p);
+                           kout, vout, c->n, shift, c->hist.p, c->any_flag.p, dn);
     }
+    if (batch)
+        hipLaunchKernelGGL(k_expand_sorted, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p, c->flags.p,
+                           c->any_flag.p, c->coff.p, c->sorted_keys(), c->sorted_perm(), kf, pf);
     hipLaunchKernelGGL(k_cell_start, dim3((c->nkeys + 1 + 255) / 256), dim3(256), 0, s, kf, c->n,
                        c->nkeys, c->start.p, c->any_flag.p);
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p, c->flags.p,
-                       c->any_flag.p, c->sorted_perm(), c->pos.p, c->vel.p, c->pidx.p, c->pos_alt.p,
+                       c->any_flag.p, pf, c->pos.p, c->vel.p, c->pidx.p, c->pos_alt.p,
                        c->vel_alt.p, c->pidx_alt.p);
     hipLaunchKernelGGL(k_scatter_back, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p,
                        c->flags.p, c->any_flag.p, c->pos_alt.p, c->vel_alt.p, c->pidx_alt.p, c->pos.p,
@@ -1083,6 +1115,9 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         (rc = c->cl_count.alloc(ncases)) || (rc = c->max_len.alloc(1)) ||
         (rc = c->rebuilds.alloc(ncases + 1)) || (rc = c->pairs.alloc(kPairSlots * kPairStride)) ||
         (rc = c->pairs_scratch.alloc(kPairSlots * kPairStride)))
+        return rc;
+    if (ncases > 1 && ((rc = c->kfull.alloc(n)) || (rc = c->pfull.alloc(n)) || (rc = c->coff.alloc(ncases + 1)) ||
+                       (rc = c->nsort.alloc(2))))
         return rc;
     hipStream_t s = c->stream;
     HIP_TRY(hipMemsetAsync(c->any_flag.p, 0, 2 * sizeof(int), s));

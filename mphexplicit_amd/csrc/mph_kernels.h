@@ -283,11 +283,91 @@ k_keys(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
     vout[i] = i;
 }
 
-__global__ void __launch_bounds__(kSortThreads)
-k_radix_hist(const unsigned* __restrict__ keys, int n, int shift, unsigned* __restrict__ hist,
-             const int* __restrict__ any_flag)
+// Batches sort only the cases flagged for a rebuild.  Their particles are compacted into one
+// array in case order (coff[c]: case c's first slot there; nsort[0]: the number of keys,
+// nsort[1]: the length of their radix histogram), sorted there and expanded back into the
+// sorted keys and the permutation of all particles.  A case's keys lie in a key range of its
+// own above those of the cases before it, so a stable sort of all keys orders each flagged
+// case's particles exactly as this does and leaves the other cases in place.
+__global__ void __launch_bounds__(256)
+k_sort_offsets(const CaseConst* __restrict__ cases, int ncases, const int* __restrict__ flags,
+               const int* __restrict__ any_flag, int* __restrict__ coff, int* __restrict__ nsort)
 {
     if (no_rebuild(any_flag)) return;
+    __shared__ int part[256];
+    const int t = threadIdx.x;
+    const int chunk = (ncases + 255) / 256;
+    const int lo = min(ncases, t * chunk), hi = min(ncases, lo + chunk);
+    int s = 0;
+    for (int c = lo; c < hi; ++c)
+        if (flags[c]) s += cases[c].end - cases[c].begin;
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int c = lo; c < hi; ++c) {
+        coff[c] = run;
+        if (flags[c]) run += cases[c].end - cases[c].begin;
+    }
+    if (t == 255) {
+        const int nf = part[255];
+        nsort[0] = nf;
+        nsort[1] = kRadix * ((nf + kSortTile - 1) / kSortTile);
+    }
+}
+
+// keys of the flagged cases' particles, compacted
+__global__ void __launch_bounds__(256)
+k_keys_compact(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
+               const double4* __restrict__ pos, const int* __restrict__ flags,
+               const int* __restrict__ any_flag, const int* __restrict__ rank, const int* __restrict__ coff,
+               unsigned* __restrict__ kout, int* __restrict__ vout)
+{
+    if (no_rebuild(any_flag)) return;
+    const int2 m = bmap[blockIdx.x];
+    if (m.x < 0) return;
+    const CaseConst& C = cases[m.x];
+    const int i = m.y + (int)threadIdx.x;
+    if (i >= C.end || flags[m.x] == 0) return;
+    const double4 p = pos[i];
+    const int e = coff[m.x] + (i - C.begin);
+    kout[e] = cell_key(C, rank, p, tag_of(p.w) & 3);
+    vout[e] = i;
+}
+
+// the sorted compacted keys and sources back at the flagged cases' particle slots
+__global__ void __launch_bounds__(256)
+k_expand_sorted(const CaseConst* __restrict__ cases, const int2* __restrict__ bmap,
+                const int* __restrict__ flags, const int* __restrict__ any_flag, const int* __restrict__ coff,
+                const unsigned* __restrict__ ks, const int* __restrict__ vs, unsigned* __restrict__ kfull,
+                int* __restrict__ vfull)
+{
+    if (no_rebuild(any_flag)) return;
+    const int2 m = bmap[blockIdx.x];
+    if (m.x < 0) return;
+    const CaseConst& C = cases[m.x];
+    const int i = m.y + (int)threadIdx.x;
+    if (i >= C.end || flags[m.x] == 0) return;
+    const int e = coff[m.x] + (i - C.begin);
+    kfull[i] = ks[e];
+    vfull[i] = vs[e];
+}
+
+// dn (batches, may be null): the number of keys to sort is dn[0], counted on the device; the
+// blocks past it return, and the histogram has one column per block that sorts
+__global__ void __launch_bounds__(kSortThreads)
+k_radix_hist(const unsigned* __restrict__ keys, int n, int shift, unsigned* __restrict__ hist,
+             const int* __restrict__ any_flag, const int* __restrict__ dn)
+{
+    if (no_rebuild(any_flag)) return;
+    if (dn) n = dn[0];
+    const int nb = dn ? (n + kSortTile - 1) / kSortTile : (int)gridDim.x;
+    if ((int)blockIdx.x >= nb) return;
     __shared__ unsigned cnt[kRadix];
     const int tid = threadIdx.x;
     cnt[tid] = 0;
@@ -298,7 +378,7 @@ k_radix_hist(const unsigned* __restrict__ keys, int n, int shift, unsigned* __re
         if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & (kRadix - 1)], 1u);
     }
     __syncthreads();
-    hist[tid * gridDim.x + blockIdx.x] = cnt[tid];
+    hist[tid * nb + blockIdx.x] = cnt[tid];
 }
 
 // exclusive scan of `total` counters in place, one workgroup of 1024 threads
@@ -337,9 +417,10 @@ constexpr int kScanTile = kScanThreads * kScanItems;
 
 __global__ void __launch_bounds__(kScanThreads)
 k_scan_tile_sums(const unsigned* __restrict__ data, int total, unsigned* __restrict__ sums,
-                 const int* __restrict__ any_flag)
+                 const int* __restrict__ any_flag, const int* __restrict__ dtotal = nullptr)
 {
     if (any_flag && no_rebuild(any_flag)) return;
+    if (dtotal) total = dtotal[0];      // counted on the device (tiles past it sum to zero)
     const int base = blockIdx.x * kScanTile;
     unsigned s = 0;
     for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) s += base + k < total ? data[base + k] : 0u;
@@ -356,9 +437,10 @@ k_scan_tile_sums(const unsigned* __restrict__ data, int total, unsigned* __restr
 
 __global__ void __launch_bounds__(kScanThreads)
 k_scan_tiles(unsigned* __restrict__ data, int total, const unsigned* __restrict__ offsets,
-             const int* __restrict__ any_flag)
+             const int* __restrict__ any_flag, const int* __restrict__ dtotal = nullptr)
 {
     if (any_flag && no_rebuild(any_flag)) return;
+    if (dtotal) total = dtotal[0];
     // padded so that the 16 consecutive counters of one thread fall in different banks
     __shared__ unsigned tile[kScanTile + kScanTile / kScanItems];
     __shared__ unsigned part[kScanThreads];
@@ -394,15 +476,18 @@ k_scan_tiles(unsigned* __restrict__ data, int total, const unsigned* __restrict_
 __global__ void __launch_bounds__(kSortThreads)
 k_radix_scatter(const unsigned* __restrict__ kin, const int* __restrict__ vin,
                 unsigned* __restrict__ kout, int* __restrict__ vout, int n, int shift,
-                const unsigned* __restrict__ hist, const int* __restrict__ any_flag)
+                const unsigned* __restrict__ hist, const int* __restrict__ any_flag, const int* __restrict__ dn)
 {
     if (no_rebuild(any_flag)) return;
+    if (dn) n = dn[0];
+    const int nb = dn ? (n + kSortTile - 1) / kSortTile : (int)gridDim.x;
+    if ((int)blockIdx.x >= nb) return;
     __shared__ unsigned base[kRadix];
     __shared__ unsigned wcnt[kSortThreads / 64][kRadix];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    base[tid] = hist[tid * gridDim.x + blockIdx.x];
+    base[tid] = hist[tid * nb + blockIdx.x];
     const int tile = blockIdx.x * kSortTile;
     for (int it = 0; it < kSortItems; ++it) {
         for (int w = 0; w < kSortThreads / 64; ++w) wcnt[w][tid] = 0;
