@@ -184,7 +184,10 @@ int mph_particle_count(const mph_ctx *ctx, int case_id);
 /* Simulated time of a case. */
 int mph_get_time(const mph_ctx *ctx, int case_id, double *time);
 
-/* Copy per-particle fields of a case to the host (synchronises). */
+/* Copy per-particle fields of a case to the host (synchronises).  Replaces the arrays that
+ * main.cpp:672-834 reads for .prof/.vtk output.  Where no case has surface tension the step
+ * does not sum DensityA (it enters no force then); this call sums it first, on the current
+ * positions and lists, which gives the values the step would have produced. */
 int mph_get_fields(mph_ctx *ctx, int case_id, int order, mph_fields *out);
 
 /* calculateVirialStressAtParticle, main.cpp:2057-2302, for the current state (the
