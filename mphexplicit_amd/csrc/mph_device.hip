@@ -106,8 +106,10 @@ struct mph_ctx {
     DevBuf<unsigned> keys_a, keys_b, hist, ccount, scan_sums;
     DevBuf<unsigned> kfull;        // batches: sorted keys of all particles
     DevBuf<int> pfull, coff, nsort;  // batches: sort sources, compacted case offsets, counts
-    DevBuf<int> perm_a, perm_b, rank, cell_of_rank, cell_case, slot_case;
+    DevBuf<int> perm_a, perm_b, rank, cell_case, slot_case;
     DevBuf<int4> cl_info;
+    DevBuf<int4> stencil;          // per cell: its stencil cells' ranges in ascending rank (27 slots)
+    DevBuf<int> stencil_rank;      // per cell: its stencil cells' ranks, ascending (fixed grid)
     DevBuf<int> cl_count, cl_nbr, cl_len;
     DevBuf<int> flags, any_flag, max_len;
     DevBuf<long long> rebuilds;
@@ -211,6 +213,42 @@ static void morton_ranks(const CaseConst& k, std::vector<int>& rank, std::vector
     for (int r = 0; r < T; ++r) {
         cell_of_rank[r] = code[r].second;
         rank[code[r].second] = r;
+    }
+}
+
+// The stencil of every cell in storage order: the Morton ranks of the cells of its 3^d window
+// ({i-1, i, i+1} per axis, periodic, in ascending cell index; one cell on an axis of range 0,
+// the 2D z axis; repeated cells where an axis has fewer than 3 cells), in ascending rank with
+// repeats in window order (main.cpp:1365-1391 visits them in ascending cell id, and cell ids
+// and ranks order a window alike).  27 entries per cell (9 used in 2D).  The grid is fixed, so
+// this is computed once; each rebuild looks up the cells' particle ranges (k_stencil).
+static void stencil_table(const CaseConst& k, const std::vector<int>& rank, const std::vector<int>& cell_of_rank,
+                          std::vector<int>& out)
+{
+    auto window = [](int ic, int c, int range, int* w) {
+        if (range == 0) { w[0] = ic; return 1; }
+        int a = (ic - 1 + c) % c, b = ic, d = (ic + 1) % c;
+        if (a > b) std::swap(a, b);
+        if (b > d) std::swap(b, d);
+        if (a > b) std::swap(a, b);
+        w[0] = a; w[1] = b; w[2] = d;
+        return 3;
+    };
+    const size_t base = out.size();
+    out.resize(base + (size_t)27 * k.tcc, 0);
+    for (int r = 0; r < k.tcc; ++r) {
+        const int cid = cell_of_rank[r];
+        const int iz = cid % k.cc[2], iy = (cid / k.cc[2]) % k.cc[1], ix = cid / (k.cc[1] * k.cc[2]);
+        int wx[3], wy[3], wz[3];
+        const int nx = window(ix, k.cc[0], 1, wx), ny = window(iy, k.cc[1], 1, wy);
+        const int nz = window(iz, k.cc[2], k.dim == 2 ? 0 : 1, wz);
+        std::vector<std::pair<int, int>> cells;
+        for (int a = 0; a < nx; ++a)
+            for (int b = 0; b < ny; ++b)
+                for (int d = 0; d < nz; ++d)
+                    cells.push_back({rank[(wx[a] * k.cc[1] + wy[b]) * k.cc[2] + wz[d]], (int)cells.size()});
+        std::sort(cells.begin(), cells.end());
+        for (size_t q = 0; q < cells.size(); ++q) out[base + (size_t)27 * r + q] = cells[q].first;
     }
 }
 
@@ -505,7 +543,7 @@ static int launch_lists(mph_ctx* c, int count_only)
     // single case the autotuner times the grids of kListGrids and keeps the fastest.
     const int lgrid = list_grid(c);
     LAUNCH_K(k_cluster_lists, c->K, dim3((unsigned)lgrid), dim3(256), 0, c->stream, c->d_cases.p,
-             c->slot_case.p, c->nslots, c->flags.p, c->any_flag.p, c->pos.p, c->start.p, c->rank.p,
+             c->slot_case.p, c->nslots, c->flags.p, c->any_flag.p, c->pos.p, c->stencil.p,
              c->cl_info.p, c->cl_count.p, count_only ? 0 : c->width, c->cl_nbr.p, c->cl_len.p, c->selfin.p, c->nout.p,
              c->max_len.p);
     return launch_ok("lists");
@@ -550,6 +588,8 @@ static int launch_rebuild(mph_ctx* c, int count_only)
                            c->any_flag.p, c->coff.p, c->sorted_keys(), c->sorted_perm(), kf, pf);
     hipLaunchKernelGGL(k_cell_start, dim3((c->nkeys + 1 + 255) / 256), dim3(256), 0, s, kf, c->n,
                        c->nkeys, c->start.p, c->any_flag.p);
+    hipLaunchKernelGGL(k_stencil, dim3((c->ncells + 7) / 8), dim3(256), 0, s, c->d_cases.p, c->cell_case.p,
+                       c->ncells, c->stencil_rank.p, c->start.p, c->flags.p, c->any_flag.p, c->stencil.p);
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(kBlock), 0, s, c->d_cases.p, c->d_bmap.p, c->flags.p,
                        c->any_flag.p, pf, c->pos.p, c->vel.p, c->pidx.p, c->pos_alt.p,
                        c->vel_alt.p, c->pidx_alt.p);
@@ -561,7 +601,7 @@ static int launch_rebuild(mph_ctx* c, int count_only)
                        c->ncells, c->start.p, c->K, c->ccount.p, c->any_flag.p);
     launch_scan(c, c->ccount.p, c->ncells);
     hipLaunchKernelGGL(k_cluster_fill, dim3(gcells), dim3(256), 0, s, c->d_cases.p, c->cell_case.p,
-                       c->ncells, c->start.p, c->cell_of_rank.p, c->K, c->ccount.p, c->cl_info.p,
+                       c->ncells, c->start.p, c->K, c->ccount.p, c->cl_info.p,
                        c->cl_count.p, c->any_flag.p);
     return launch_lists(c, count_only);
 }
@@ -1033,7 +1073,7 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     std::vector<int> hidx(c->n);
     std::vector<double> hwall((size_t)ncases * kTypes * 3, 0.0);
     int begin = 0, key_base = 0;
-    std::vector<int> hrank, hcell, hcell_case;
+    std::vector<int> hrank, hcell_case, hstencil;
     std::vector<int2> logical;
     c->any_surface = 0;
     c->same_pv = 1;
@@ -1082,9 +1122,9 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         }
         std::vector<int> rk, cr;
         morton_ranks(ch.k, rk, cr);
+        stencil_table(ch.k, rk, cr, hstencil);
         ch.k.rank_base = (int)hrank.size();
         hrank.insert(hrank.end(), rk.begin(), rk.end());
-        hcell.insert(hcell.end(), cr.begin(), cr.end());
         hcell_case.insert(hcell_case.end(), (size_t)ch.k.tcc, k);
         for (int b = 0; b < nk; b += kBlock) logical.push_back(make_int2(k, begin + b));
         begin += nk;
@@ -1109,8 +1149,9 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
         (rc = c->wallc.alloc(hwall.size())) || (rc = c->keys_a.alloc(n)) || (rc = c->keys_b.alloc(n)) ||
         (rc = c->hist.alloc((size_t)kRadix * c->sort_blocks)) || (rc = c->perm_a.alloc(n)) ||
         (rc = c->perm_b.alloc(n)) || (rc = c->flags.alloc(ncases)) || (rc = c->any_flag.alloc(2)) ||
-        (rc = c->rank.alloc(hrank.size())) || (rc = c->cell_of_rank.alloc(hcell.size())) ||
+        (rc = c->rank.alloc(hrank.size())) ||
         (rc = c->cell_case.alloc(c->ncells)) || (rc = c->ccount.alloc(c->ncells)) ||
+        (rc = c->stencil.alloc((size_t)27 * c->ncells)) || (rc = c->stencil_rank.alloc(hstencil.size())) ||
         (rc = c->scan_sums.alloc((size_t)std::max(kRadix * c->sort_blocks, c->ncells) / kScanTile + 2)) ||
         (rc = c->cl_count.alloc(ncases)) || (rc = c->max_len.alloc(1)) ||
         (rc = c->rebuilds.alloc(ncases + 1)) || (rc = c->pairs.alloc(kPairSlots * kPairStride)) ||
@@ -1127,7 +1168,8 @@ static int setup(mph_ctx* c, int ncases, const mph_params* prm, const int* count
     HIP_TRY(hipMemcpyAsync(c->pidx.p, hidx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->wallc.p, hwall.data(), sizeof(double) * hwall.size(), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->rank.p, hrank.data(), sizeof(int) * hrank.size(), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->cell_of_rank.p, hcell.data(), sizeof(int) * hcell.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->stencil_rank.p, hstencil.data(), sizeof(int) * hstencil.size(), hipMemcpyHostToDevice,
+                           s));
     HIP_TRY(hipMemcpyAsync(c->cell_case.p, hcell_case.data(), sizeof(int) * hcell_case.size(), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(c->aux.p, 0, sizeof(double4) * n, s));
     HIP_TRY(hipMemsetAsync(c->force.p, 0, sizeof(double4) * n, s));

@@ -594,7 +594,7 @@ k_scatter_back(const CaseConst* __restrict__ cases, const int2* __restrict__ bma
 // outside a member's cut-offs contribute exactly zero (the kernel factors vanish there),
 // so each member's sums cover exactly the pairs of its reference list that interact.
 //
-// Cluster table entry: int4 {first particle, size, reference cell id, case}.
+// Cluster table entry: int4 {first particle, size, storage slot of its cell, case}.
 
 // the periodic window {ic-1, ic, ic+1} (or {ic} for range 0) in ascending order,
 // repeats included when the axis has fewer than 3 cells (as main.cpp:1365-1391)
@@ -650,7 +650,7 @@ k_cluster_count(const CaseConst* __restrict__ cases, const int* __restrict__ cel
 // cluster table from the scanned counts; non-rebuilt cases are rewritten with the same values
 __global__ void __launch_bounds__(256)
 k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell_case, int ncells,
-               const int* __restrict__ start, const int* __restrict__ cell_of_rank, int K,
+               const int* __restrict__ start, int K,
                const unsigned* __restrict__ coff, int4* __restrict__ cl_info, int* __restrict__ cl_count,
                const int* __restrict__ any_flag)
 {
@@ -664,10 +664,9 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
     const int first = start[k0], n = start[k0 + 2] - first;
     const int base = (int)(coff[s] - coff[C.rank_base]);
     const int nq = (n + K - 1) / K;
-    const int cid = cell_of_rank[s];
     for (int q = 0; q < nq; ++q) {
         const int g = base + q;
-        if (g < C.cl_cap) cl_info[C.cl_base + g] = make_int4(first + q * K, min(K, n - q * K), cid, c);
+        if (g < C.cl_cap) cl_info[C.cl_base + g] = make_int4(first + q * K, min(K, n - q * K), s, c);
     }
     if (r == C.tcc - 1) cl_count[c] = base + nq;
 }
@@ -697,6 +696,36 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
 // reference's arithmetic on the original positions.  list_img (host) bounds the rounding of
 // both forms below that band.  Other wavefronts use the reference's arithmetic per pair.
 
+// Stencil records, once per rebuilt cell: the particle ranges of the cell's stencil cells in
+// ascending rank (the table stencil_rank, fixed with the grid), as {first index, end of the
+// fluid part, count, exclusive prefix of the counts}, at stencil[slot * 27 + q] for the cell's
+// storage slot.  The list build's wavefronts of a cell (about six per cell) read them instead
+// of each ranking the stencil and loading its ranges.  Two cells per wavefront (lanes 0-31 and
+// 32-63); cells of cases without a rebuild keep their records (their ranges do not move).
+__global__ void __launch_bounds__(256)
+k_stencil(const CaseConst* __restrict__ cases, const int* __restrict__ cell_case, int ncells,
+          const int* __restrict__ stencil_rank, const int* __restrict__ start, const int* __restrict__ flags,
+          const int* __restrict__ any_flag, int4* __restrict__ stencil)
+{
+    if (no_rebuild(any_flag)) return;
+    const int q = threadIdx.x & 31;
+    const int slot = blockIdx.x * 8 + (threadIdx.x >> 5);
+    if (slot >= ncells) return;
+    const int c = cell_case[slot];
+    if (flags[c] == 0) return;
+    const CaseConst& C = cases[c];
+    const int nc = C.dim == 2 ? 9 : 27;
+    const int r = stencil_rank[(size_t)slot * 27 + (q < nc ? q : 0)];
+    const int k0 = C.key_base + 2 * r;
+    const int lo = start[k0], mid = start[k0 + 1], cnt = q < nc ? start[k0 + 2] - lo : 0;
+    int pre = cnt;                               // inclusive prefix over the half-wavefront
+    for (int off = 1; off < 32; off <<= 1) {
+        const int t = __shfl_up(pre, off, 32);
+        if (q >= off) pre += t;
+    }
+    if (q < nc) stencil[(size_t)slot * 27 + q] = make_int4(lo, mid, cnt, pre - cnt);
+}
+
 // Ring of the candidates that passed the box test, in planes (x, y | z | index with the
 // fluid flag in bit 31), and the members of the wavefront's clusters (x, y, z, index): the
 // ring's planes are read and written at consecutive entries, the members by broadcast
@@ -705,7 +734,7 @@ struct ListLds {
     double z[128];
     int jf[128];
     double4 mem[8];
-    int srank[32], clo[32], cmid[32], cpre[33];
+    int clo[32], cmid[32], cpre[33];
 };
 
 // wave-wide ballot of a lane predicate straight into a scalar mask
@@ -718,30 +747,15 @@ __device__ __forceinline__ unsigned long long ballot64(bool p) { return __builti
 // a member's hits are one ballot and the union of a cluster the OR of its members' ballots.
 template <int K, int P>
 __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const int4 info, int g1, const int4 info1,
-                                                 const double4* __restrict__ pos, const int* __restrict__ start,
-                                                 const int* __restrict__ rank, int width, int* __restrict__ cl_nbr,
+                                                 const double4* __restrict__ pos, const int4* __restrict__ stencil,
+                                                 int width, int* __restrict__ cl_nbr,
                                                  int* __restrict__ cl_len, int* __restrict__ selfin,
                                                  int* __restrict__ count_out, ListLds& L)
 {
     constexpr int M = K * P;                           // member slots
     const int lane = threadIdx.x & 63;
-    // stencil cells: lane q < nc takes the q-th cell of the window (its rank and particle
-    // range are loaded first, so that this chain of loads overlaps the members' loads and
-    // box below), then its place in ascending rank (repeated cells, on axes of fewer than 3
-    // cells, in lane order)
-    const int cid = info.z;
-    const int iz = cid % C.cc[2];
-    const int iy = (cid / C.cc[2]) % C.cc[1];
-    const int ix = cid / (C.cc[1] * C.cc[2]);
-    int wx[3], wy[3], wz[3];
-    const int ncx = axis_window(ix, C.cc[0], 1, wx);
-    const int ncy = axis_window(iy, C.cc[1], 1, wy);
-    const int ncz = axis_window(iz, C.cc[2], C.dim == 2 ? 0 : 1, wz);
-    const int nc = ncx * ncy * ncz;
-    const int qc = lane < nc ? lane : 0;
-    const int v = rank[C.rank_base + (wx[qc / (ncy * ncz)] * C.cc[1] + wy[(qc / ncz) % ncy]) * C.cc[2] + wz[qc % ncz]];
-    const int sk = C.key_base + 2 * v;
-    const int slo = start[sk], smid = start[sk + 1], shi = start[sk + 2];
+    const int cell = info.z;                           // storage slot of the cluster's cell
+    const int nc = C.dim == 2 ? 9 : 27;
     // lane l < M takes member slot l; the other lanes repeat the slots (l mod M), and slots
     // of absent members repeat the cluster's first member (which does not change the box)
     const int ms = lane % M;
@@ -795,27 +809,17 @@ __device__ __forceinline__ int cluster_list_one(const CaseConst& C, int g, const
     const double w8 = 64.0 * C.cut2;
     const bool use_box = C.dwid[0] * C.dwid[0] >= w8 && C.dwid[1] * C.dwid[1] >= w8 &&
                          (C.dim == 2 || C.dwid[2] * C.dwid[2] >= w8);
-    if (lane < nc) L.srank[lane] = v;
-    __builtin_amdgcn_wave_barrier();
+    // the stencil cells' particle ranges, ascending in rank, from the cell's record
+    // (k_stencil): lane q < nc holds cell q's first index, fluid end, count and prefix.  Loaded
+    // here, not earlier: held across the members' box it pushed the kernel into spilling
+    __builtin_amdgcn_sched_barrier(0);
+    const int4 srec = stencil[(size_t)cell * 27 + (lane < nc ? lane : 0)];
     if (lane < nc) {
-        int at = 0;
-        for (int q = 0; q < nc; ++q) {
-            const int u = L.srank[q];
-            at += u < v || (u == v && q < lane);
-        }
-        L.clo[at] = slo;
-        L.cmid[at] = smid;
-        L.srank[at] = shi - slo;               // the counts in ascending rank (srank is read above)
+        L.clo[lane] = srec.x;
+        L.cmid[lane] = srec.y;
+        L.cpre[lane] = srec.w;
     }
-    __builtin_amdgcn_wave_barrier();
-    // exclusive prefix of the counts over the cells in ascending rank
-    int pre = lane < nc ? L.srank[lane] : 0;
-    for (int off = 1; off < 32; off <<= 1) {
-        const int t = __shfl_up(pre, off);
-        if (lane >= off) pre += t;
-    }
-    if (lane < nc) L.cpre[lane + 1] = pre;
-    if (lane == 0) L.cpre[0] = 0;
+    if (lane == nc - 1) L.cpre[nc] = srec.w + srec.z;
     __builtin_amdgcn_wave_barrier();
     const int total = L.cpre[nc];
     // a block's lanes e = base + lane take the cell q with cpre[q] <= e: from the block's first
@@ -1026,8 +1030,8 @@ template <int K>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slot_case, int nslots,
                 const int* __restrict__ flags, int* __restrict__ any_flag,
-                const double4* __restrict__ pos, const int* __restrict__ start,
-                const int* __restrict__ rank, const int4* __restrict__ cl_info,
+                const double4* __restrict__ pos, const int4* __restrict__ stencil,
+                const int4* __restrict__ cl_info,
                 const int* __restrict__ cl_count, int width, int* __restrict__ cl_nbr,
                 int* __restrict__ cl_len, int* __restrict__ selfin, int* __restrict__ count_out,
                 int* __restrict__ max_len)
@@ -1040,7 +1044,7 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
     int longest = 0;
     const int4 none = make_int4(0, 0, 0, 0);
 #define MPH_LIST_ONE(PP, CC, G, I, G1, I1)                                                                  \
-    cluster_list_one<K, PP>(cases[CC], G, I, G1, I1, pos, start, rank, width, cl_nbr, cl_len, selfin,       \
+    cluster_list_one<K, PP>(cases[CC], G, I, G1, I1, pos, stencil, width, cl_nbr, cl_len, selfin,          \
                             count_out, lds[wv])
     if (K <= 4) {
         // slots in pairs (2t, 2t + 1): two clusters of one cell share a wavefront
