@@ -673,7 +673,8 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
 
 // Union lists, one wavefront per cluster, or per two clusters of one cell when K <= 4
 // (main.cpp:1329-1596 per member, united in ascending index).  The candidates are the
-// particles of the cell's 3^d stencil cells in ascending rank and index.  The wavefront takes
+// particles of the cell's 3^d stencil cells in ascending rank and index (the cell's record
+// from k_stencil).  The wavefront takes
 // 64 at a time (one load per lane, the next block's loads in flight), drops those farther than
 // the cut-off from the members' bounding box (conservatively: cut-off^2 (1 + 1e-8) against a
 // box built from the same differences as the pair test), and appends the rest to a ring of
@@ -686,7 +687,8 @@ k_cluster_fill(const CaseConst* __restrict__ cases, const int* __restrict__ cell
 // rebuild of a few clusters runs them in parallel.  Full rebuilds of the 1.27M-particle deck:
 // one thread per member walking the candidates with a dependent load each, 5.5-6 ms; rows
 // (members) x slots testing each 64-candidate block as soon as it was compacted (partly
-// filled steps, the union appended per step), 1.84 ms; this kernel, 1.19 ms.
+// filled steps, the union appended per step), 1.84 ms; this kernel, 1.07 ms at six wavefronts
+// per SIMD with the stencil records.
 //
 // Pair decisions.  A wavefront whose members are all at least `guard` inside both faces of
 // an axis has no list pair that wraps on that axis.  On such axes, and on the axes near a face
