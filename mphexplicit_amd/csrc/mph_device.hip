@@ -524,11 +524,15 @@ static void launch_scan_dev(mph_ctx* c, unsigned* data, int max_total, const int
                        c->any_flag.p, dtotal);
 }
 
+// work items per XCD chunk of a single case's list build (0: no XCD chunks)
+constexpr int kListChunk = 256;
+
 // workgroups of the list build: the autotuner's choice (c->lcap), else the default cap
 static int list_grid(const mph_ctx* c)
 {
     const int lcap = c->lcap > 0 ? c->lcap : (c->ncases == 1 ? 20480 : 4096);
-    return std::max(1, std::min((c->nslots + 3) / 4, lcap));
+    const int g = std::max(1, std::min((c->nslots + 3) / 4, lcap));
+    return (g + kXcds - 1) / kXcds * kXcds;          // the kernel's XCD chunks want a multiple of 8
 }
 
 // the union lists of the cases flagged for a rebuild (count_only: only their lengths)
@@ -545,7 +549,7 @@ static int launch_lists(mph_ctx* c, int count_only)
     LAUNCH_K(k_cluster_lists, c->K, dim3((unsigned)lgrid), dim3(256), 0, c->stream, c->d_cases.p,
              c->slot_case.p, c->nslots, c->flags.p, c->any_flag.p, c->pos.p, c->stencil.p,
              c->cl_info.p, c->cl_count.p, count_only ? 0 : c->width, c->cl_nbr.p, c->cl_len.p, c->selfin.p, c->nout.p,
-             c->max_len.p);
+             c->max_len.p, c->ncases == 1 ? kListChunk : 0);
     return launch_ok("lists");
 }
 

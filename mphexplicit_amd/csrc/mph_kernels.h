@@ -1036,13 +1036,26 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
                 const int4* __restrict__ cl_info,
                 const int* __restrict__ cl_count, int width, int* __restrict__ cl_nbr,
                 int* __restrict__ cl_len, int* __restrict__ selfin, int* __restrict__ count_out,
-                int* __restrict__ max_len)
+                int* __restrict__ max_len, int chunk)
 {
     if (no_rebuild(any_flag)) return;
     __shared__ ListLds lds[4];
     // the wavefront's slot and case are uniform: readfirstlane lets the compiler keep the
     // case's constants in scalar registers (scalar loads) instead of vector loads in the loop
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // chunk > 0 (and a grid of a multiple of 8): the work items (pairs of slots, or slots) go
+    // in chunks of `chunk` consecutive items to the XCDs in turn, and workgroup b, which runs
+    // on XCD b % 8, walks that XCD's chunks in order: the stencil cells' positions a wavefront
+    // reads are then mostly in its XCD's L2 already.  Otherwise the workgroups stride over all
+    // items in turn (a batch's flagged slots can lie in one region: chunks would idle XCDs).
+    const bool xm = chunk > 0 && gridDim.x % kXcds == 0;
+    const int nt = K <= 4 ? (nslots + 1) / 2 : nslots;
+    const int C = xm ? chunk : max(nt, 1);
+    const int nx = xm ? kXcds : 1;
+    const int x = xm ? (int)(blockIdx.x % kXcds) : 0;
+    const int lb = xm ? (int)(blockIdx.x / kXcds) : (int)blockIdx.x;
+    const int step = 4 * (int)gridDim.x / nx;
+    const int uend = ((nt + C - 1) / C - x + nx - 1) / nx * C;        // items of this XCD's chunks
     int longest = 0;
     const int4 none = make_int4(0, 0, 0, 0);
 #define MPH_LIST_ONE(PP, CC, G, I, G1, I1)                                                                  \
@@ -1050,7 +1063,9 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
                             count_out, lds[wv])
     if (K <= 4) {
         // slots in pairs (2t, 2t + 1): two clusters of one cell share a wavefront
-        for (int t = blockIdx.x * 4 + wv; 2 * t < nslots; t += gridDim.x * 4) {
+        for (int u = lb * 4 + wv; u < uend; u += step) {
+            const int t = (u / C * nx + x) * C + u % C;     // grows with u: past the end, stop
+            if (t >= nt) break;
             const int g0 = 2 * t, g1 = g0 + 1;
             // the cluster entries are loaded with the slots' cases, not after them
             const int4 i0 = cl_info[g0], i1 = cl_info[g1 < nslots ? g1 : g0];
@@ -1070,7 +1085,9 @@ k_cluster_lists(const CaseConst* __restrict__ cases, const int* __restrict__ slo
             if (v1) longest = max(longest, MPH_LIST_ONE(1, c1, g1, i1, 0, none));
         }
     } else {
-        for (int g = blockIdx.x * 4 + wv; g < nslots; g += gridDim.x * 4) {
+        for (int u = lb * 4 + wv; u < uend; u += step) {
+            const int g = (u / C * nx + x) * C + u % C;
+            if (g >= nt) break;
             const int c = __builtin_amdgcn_readfirstlane(slot_case[g]);
             if (flags[c] == 0 || g - cases[c].cl_base >= cl_count[c]) continue;     // wave-uniform
             longest = max(longest, MPH_LIST_ONE(1, c, g, cl_info[g], 0, none));
