@@ -852,7 +852,8 @@ static int check_capacity(mph_ctx* c)
 }
 
 // Cluster slots, block map and lists for cluster size K: each case reserves
-// ceil(n/K) + cells slots (every cell adds at most one partial cluster).  All cases are
+// ceil(n/K) + cells slots (every cell adds at most one partial cluster), rounded up to a
+// multiple of 4.  All cases are
 // flagged for a rebuild, the union lists are counted to size their width, then built.
 static int prepare_clusters(mph_ctx* c, int K)
 {
@@ -864,9 +865,11 @@ static int prepare_clusters(mph_ctx* c, int K)
         CaseConst& kc = c->cases[k].k;
         const int nk = kc.end - kc.begin;
         kc.cl_base = base;
-        kc.cl_cap = (nk + K - 1) / K + kc.tcc;
+        // a multiple of 4: the four slots of a map entry stay inside the case (cluster_frame
+        // loads a slot's entry before it tests the slot against the count)
+        kc.cl_cap = ((nk + K - 1) / K + kc.tcc + 3) / 4 * 4;
         for (int q = 0; q < kc.cl_cap; ++q) hslot_case.push_back(k);
-        for (int q = 0; q < kc.cl_cap; q += 4) logical.push_back(make_int2(k, q));
+        for (int q = 0; q < kc.cl_cap; q += 4) logical.push_back(make_int2(k, base + q));   // global slot
         base += kc.cl_cap;
     }
     c->nslots = base;

@@ -1132,26 +1132,49 @@ struct Frame {
     int first;      // first member (reference point of the mixed-precision sweeps)
     bool valid;     // row < cluster size
     const int* list;
+    int width;      // entries reserved for the list
+    int j0;         // list entry `lane` as loaded (the length not yet applied)
 };
 
-template <int K>
+// The wavefront's cluster.  first_slot is the global slot of the block's first cluster (the
+// block map's entry).  PIN: the slot's entry, union length and first 64 list entries (j0) are
+// loaded together with the case's cluster count, and only then is the slot tested against the
+// count; the four slots of a map entry lie inside the case's capacity (a multiple of 4), so
+// the loads of an unused slot stay in bounds.  The empty asm takes the loaded values, so the
+// compiler issues every load before the test instead of sinking them behind it: one round
+// trip instead of two (sweep A without surface tension: 0.745 -> 0.733 ms on the 1.27M deck).
+// Otherwise the loads follow the test and j0 is not loaded: sweep B and the mixed-precision
+// sweeps, at their register budgets, measured slower with the loads held across the test.
+template <int K, bool PIN = false>
 __device__ __forceinline__ bool cluster_frame(const CaseConst& C, int first_slot, const int4* __restrict__ cl_info,
                                               const int* __restrict__ cl_count, int case_id,
                                               const int* __restrict__ cl_nbr, const int* __restrict__ cl_len,
                                               int width, Frame<K>& f)
 {
-    const int local = first_slot + (int)(threadIdx.x >> 6);
-    if (local >= cl_count[case_id]) return false;        // wave-uniform
-    f.g = C.cl_base + local;
-    const int4 info = cl_info[f.g];
     const int lane = threadIdx.x & 63;
+    f.g = first_slot + (int)(threadIdx.x >> 6);
+    f.width = width;
+    f.list = cl_nbr + (size_t)f.g * width;
+    int4 info;
+    if (PIN) {
+        const int count = cl_count[case_id];
+        info = cl_info[f.g];
+        f.len = cl_len[f.g];
+        f.j0 = lane < width ? f.list[lane] : -1;
+        const int base = C.cl_base;
+        asm volatile("" ::"s"(count), "s"(base), "s"(info.x), "s"(info.y), "s"(f.len), "v"(f.j0));
+        if (f.g - base >= count) return false;          // wave-uniform
+    } else {
+        if (f.g - C.cl_base >= cl_count[case_id]) return false;
+        info = cl_info[f.g];
+        f.len = cl_len[f.g];
+        f.j0 = -1;
+    }
     f.row = lane / Frame<K>::S;
     f.slot = lane % Frame<K>::S;
     f.valid = f.row < info.y;
     f.i = info.x + (f.valid ? f.row : 0);
     f.first = info.x;
-    f.len = cl_len[f.g];
-    f.list = cl_nbr + (size_t)f.g * width;
     return true;
 }
 
@@ -1198,7 +1221,7 @@ struct NoPrep {
     __device__ void operator()(Rec&) const {}
 };
 
-template <int K, int NREC, class Fetch, class Eval, bool PL3 = false, class Prep = NoPrep>
+template <int K, int NREC, class Fetch, class Eval, bool PL3 = false, class Prep = NoPrep, bool J0 = false>
 __device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restrict__ lds, Fetch fetch, Eval eval,
                                                Prep prep = Prep())
 {
@@ -1213,7 +1236,8 @@ __device__ __forceinline__ void walk_union_lds(const Frame<K>& f, v2d* __restric
     Rec r = far;
     // indices run two blocks ahead of the records, so a block's record loads never wait
     // on its index load
-    int jr = idx(0), jn = idx(1);
+    // J0: block 0's entries came with the (pinned) frame; otherwise they are loaded here
+    int jr = J0 ? (lane < len ? f.j0 : -1) : idx(0), jn = idx(1);
     if (jr >= 0) fetch(jr, r);
     for (int blk = 0; blk < blocks; ++blk) {
         // LDS operations of one wavefront complete in issue order: the reads of the
@@ -1387,7 +1411,8 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
         const double ud = fma(q.v.z - viz, dz, fma(q.v.y - viy, dy, (q.v.x - vix) * dx));
         acc.ds = fma(ud, omp * ir, acc.ds);
     };
-    walk_union_lds<K, 2, decltype(fetch), decltype(eval), R1>(f, lds, fetch, eval, ImagePrep<WRAP == 1>{C, pf, axes});
+    walk_union_lds<K, 2, decltype(fetch), decltype(eval), R1, ImagePrep<WRAP == 1>, true>(f, lds, fetch, eval,
+                                                                                        ImagePrep<WRAP == 1>{C, pf, axes});
 }
 
 // Without surface tension both sweeps run at six wavefronts per SIMD (at most 80 VGPRs, a
@@ -1415,21 +1440,30 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
 {
     // no side effect beyond its outputs: the eager step queues it before it knows whether the
     // lists are rebuilt (speculative) and, if they are, once more after the rebuild; the
-    // speculative launch returns at once when the move raised the rebuild flag
-    if (halted(any_flag) || (speculative && any_flag[0] != 0)) return;
+    // speculative launch returns at once when the move raised the rebuild flag.  The block's
+    // map entry is loaded with the flags (one round trip for both).
     const int2 m = cmap[blockIdx.x];
-    if (m.x < 0) return;
+    const int flag0 = any_flag[0], flag1 = any_flag[1];
+    asm volatile("" ::"s"(m.x), "s"(m.y), "s"(flag0), "s"(flag1));
+    if (flag1 != 0 || (speculative && flag0 != 0) || m.x < 0) return;
     const CaseConst& C = cases[m.x];
+    // the InteractionRatio table: only the n^A sums weigh pairs by it
+    constexpr bool TAB = ((MODE & 4) == 0 || SURF) && !((MODE & 2) != 0 && !SURF);
     __shared__ double tab[32];                // [0,16): mu_ij, [16,32): InteractionRatio
     constexpr int S = Frame<K>::S;
     __shared__ v2d lds[4][4 * 64];        // per wavefront: {pos, vel} halves x 64 entries
-    if (threadIdx.x < 16) {
-        tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
-        tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
-    }
-    __syncthreads();
+    // the frame's loads are in flight while the table is filled; wavefronts of unused slots
+    // leave after the barrier
     Frame<K> f;
-    if (!cluster_frame<K>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f)) return;
+    const bool live = cluster_frame<K, true>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f);
+    if (TAB) {
+        if (threadIdx.x < 16) {
+            tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
+            tab[16 + threadIdx.x] = C.ir[threadIdx.x >> 2][threadIdx.x & 3];
+        }
+        __syncthreads();
+    }
+    if (!live) return;
     const int w = threadIdx.x >> 6;
     const double4 pi = pos[f.i];
     const int ti = tag_of(pi.w) & 3;
