@@ -153,6 +153,18 @@ __device__ __forceinline__ double rsqrt_pair3(double q2)
 constexpr double kFar = 1.0e30;
 __device__ __forceinline__ double far_if_self(double q2) { return q2 > 0.0 ? q2 : 1.0e300; }
 
+// a per-type case constant for a lane's type t in 0..3: four scalar loads and selects instead
+// of a vector load indexed by the lane's type (one round trip fewer at a wavefront's end)
+__device__ __forceinline__ double by_type(const double (&a)[kTypes], int t)
+{
+    static_assert(kTypes == 4, "four particle types");
+    double a0 = a[0], a1 = a[1], a2 = a[2], a3 = a[3];
+    // opaque to the compiler, which would otherwise fold the selects into one vector load
+    // from a selected address
+    asm("" : "+s"(a0), "+s"(a1), "+s"(a2), "+s"(a3));
+    return t == 0 ? a0 : (t == 1 ? a1 : (t == 2 ? a2 : a3));
+}
+
 // The step's device flags, any_flag[0..1]:
 //   [0] some case rebuilds its lists this step (the Verlet trigger, raised by the move)
 //   [1] a list build of this step outgrew the reserved list width.  Every later kernel then
@@ -1468,6 +1480,8 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     const double4 pi = pos[f.i];
     const int ti = tag_of(pi.w) & 3;
     const double vix = vel[f.i].x, viy = vel[f.i].y, viz = vel[f.i].z;
+    // the epilogue's per-particle words, loaded with the row's own data (landed by the end)
+    const int own_i = selfin[f.i], listed_i = count_out[f.i];
     AccA acc = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
     // a wavefront takes a periodic path if any of its rows needs it (uniform branch)
     int axes;
@@ -1495,19 +1509,19 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     int npair = COUNT ? slot_sum_i<K>(acc.np) : 0;
     if (f.slot == 0 && f.valid) {
         const int i = f.i;
-        const int own = selfin[i];             // the row's own entry: w_P(0) = 1 and one pair
+        const int own = own_i;                 // the row's own entry: w_P(0) = 1 and one pair
         acc.ws -= (double)own;
         npair -= own;
-        if (count_out[i] == 0)                 // no list (main.cpp:1416): all sums are zero
+        if (listed_i == 0)                     // no list (main.cpp:1416): all sums are zero
             acc = AccA{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0};
         const double na = acc.na * C.inv_ha * C.ca;
         const double ep = C.cp * acc.ws - C.n0p;
         const double dv = -C.dcp * acc.ds;
-        const double kap = ep < 0.0 ? 0.0 : C.kbulk[ti];
-        const double p = -C.lbulk[ti] * dv + kap * ep;
+        const double kap = ep < 0.0 ? 0.0 : by_type(C.kbulk, ti);
+        const double p = -by_type(C.lbulk, ti) * dv + kap * ep;
         double pa = 0.0;
         if (NA) {
-            pa = C.cofa[ti] * (na - C.n0a) / C.l0;
+            pa = by_type(C.cofa, ti) * (na - C.n0a) / C.l0;
             if (C.n0a <= na) pa = 0.0;
         }
         const double gs = C.cg * C.g_scale;
@@ -1692,13 +1706,13 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     double fx = acc.px, fy = acc.py, fz = acc.pz;       // pressure + viscosity
     if (SURF) {
         const double ca = C.dca * C.vol;
-        const double cs = C.adiff[ti] * C.s_scale;
+        const double cs = by_type(C.adiff, ti) * C.s_scale;
         fx += acc.ax * ca; fy += acc.ay * ca; fz += acc.az * ca;
         fx += acc.sx * cs; fy += acc.sy * cs; fz += acc.sz * cs;
     }
     double4 vn = make_double4(vi.x, vi.y, vi.z, 0.0);
     if (ti < 2) {
-        const double ms = C.mass[ti];
+        const double ms = by_type(C.mass, ti);
         fx += ms * C.grav[0];
         fy += ms * C.grav[1];
         fz += ms * C.grav[2];
