@@ -142,6 +142,9 @@ struct mph_ctx {
     int* sorted_perm() { return (sort_passes & 1) ? perm_b.p : perm_a.p; }
     int nblocks() const { return (int)bmap_host.size(); }
     int ncblocks() const { return (int)cmap_host.size(); }
+    // sweep A without surface tension: eight-wavefront workgroups, each two consecutive entries
+    // of one XCD's run
+    int nablocks() const { return ((ncblocks() - kXcds) / kXcds + 1) / 2 * kXcds; }
     double4* vnext_out(int real) { return real ? vnext.p : tune_v.p; }
     double4* force_out(int real) { return real ? force.p : tune_f.p; }
 };
@@ -640,7 +643,7 @@ static int launch_sweep_a(mph_ctx* c, int real, bool count = false, bool specula
     }
     const int mode = (c->same_ap ? 1 : 0) | (c->ratio_one ? 2 : 0);
 #define SWEEP_A_ARGS                                                                                        \
-    dim3(c->ncblocks()), dim3(256), 0, c->stream, c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p,  \
+    dim3(c->any_surface ? c->ncblocks() : c->nablocks()), dim3(c->any_surface ? 256 : 512), 0, c->stream, c->d_cases.p, c->d_cmap.p, c->cl_info.p, c->cl_count.p,  \
         c->cl_nbr.p, c->cl_len.p, c->width, c->nout.p, c->pos.p, c->vel.p, c->aux.p, c->dens_a.p,          \
         c->vstrain.p, c->divp.p, real ? c->pairs.p : c->pairs_scratch.p, c->selfin.p, c->any_flag.p, spec
     const bool cnt = count && real;
@@ -874,6 +877,8 @@ static int prepare_clusters(mph_ctx* c, int K)
     }
     c->nslots = base;
     c->cmap_host = xcd_order(logical);
+    // eight padding entries: sweep A's workgroups take entries in pairs (k_sweep_a)
+    for (int q = 0; q < kXcds; ++q) c->cmap_host.push_back(make_int2(-1, 0));
     std::vector<CaseConst> hk(c->ncases);
     for (int k = 0; k < c->ncases; ++k) hk[k] = c->cases[k].k;
     int rc = 0;

@@ -1161,10 +1161,10 @@ template <int K, bool PIN = false>
 __device__ __forceinline__ bool cluster_frame(const CaseConst& C, int first_slot, const int4* __restrict__ cl_info,
                                               const int* __restrict__ cl_count, int case_id,
                                               const int* __restrict__ cl_nbr, const int* __restrict__ cl_len,
-                                              int width, Frame<K>& f)
+                                              int width, Frame<K>& f, int wave = -1)
 {
     const int lane = threadIdx.x & 63;
-    f.g = first_slot + (int)(threadIdx.x >> 6);
+    f.g = first_slot + (wave >= 0 ? wave : (int)(threadIdx.x >> 6));   // the wavefront's slot of the entry
     f.width = width;
     f.list = cl_nbr + (size_t)f.g * width;
     int4 info;
@@ -1439,8 +1439,16 @@ __device__ __forceinline__ void sweep_a_pairs(const CaseConst& C, const double* 
 // tension the attractive pressure is zero (its coefficient a_t is), so n^A changes nothing
 // in the step and is an output field only; the host computes it when the fields are read,
 // by this kernel without bit 2 on the same positions and lists (the same sums, bit for bit).
+//
+// Without surface tension, workgroups of eight wavefronts: workgroup b (on XCD b % 8) takes the
+// block map's entries (b % 8) + 8 (2 (b / 8) + h), h = 0, 1, consecutive entries of that XCD's
+// run (the map holds eight padding entries, so the last pair stays in bounds).  Eight
+// neighbouring clusters on one CU share more of their partner records in its L1: 655-656 ->
+// 664-665 steps/s at the start of the 1.27M deck's flow, the developed flow within 0.3%;
+// sweep B measured slower so (0.804 -> 0.818 ms), and one-wavefront workgroups slower for both.
+// With surface tension, four (five wavefronts per SIMD are not a multiple of two).
 template <int SURF, int K, bool COUNT, int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SURF ? 5 : 6, 8)))
+__global__ void __launch_bounds__(SURF ? 256 : 512) __attribute__((amdgpu_waves_per_eu(SURF ? 5 : 6, 8)))
 k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
           const int4* __restrict__ cl_info, const int* __restrict__ cl_count,
           const int* __restrict__ cl_nbr, const int* __restrict__ cl_len, int width,
@@ -1454,7 +1462,12 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     // lists are rebuilt (speculative) and, if they are, once more after the rebuild; the
     // speculative launch returns at once when the move raised the rebuild flag.  The block's
     // map entry is loaded with the flags (one round trip for both).
-    const int2 m = cmap[blockIdx.x];
+    constexpr int WPB = SURF ? 4 : 8;         // wavefronts per workgroup
+    const int entry = WPB == 4 ? (int)blockIdx.x
+                               : __builtin_amdgcn_readfirstlane((int)((blockIdx.x & (kXcds - 1)) +
+                                                                      kXcds * (2 * (blockIdx.x / kXcds) + (threadIdx.x >> 8))));
+    const int wave = __builtin_amdgcn_readfirstlane((int)((threadIdx.x >> 6) & 3));
+    const int2 m = cmap[entry];
     const int flag0 = any_flag[0], flag1 = any_flag[1];
     asm volatile("" ::"s"(m.x), "s"(m.y), "s"(flag0), "s"(flag1));
     if (flag1 != 0 || (speculative && flag0 != 0) || m.x < 0) return;
@@ -1463,11 +1476,11 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     constexpr bool TAB = ((MODE & 4) == 0 || SURF) && !((MODE & 2) != 0 && !SURF);
     __shared__ double tab[32];                // [0,16): mu_ij, [16,32): InteractionRatio
     constexpr int S = Frame<K>::S;
-    __shared__ v2d lds[4][4 * 64];        // per wavefront: {pos, vel} halves x 64 entries
+    __shared__ v2d lds[WPB][4 * 64];      // per wavefront: {pos, vel} halves x 64 entries
     // the frame's loads are in flight while the table is filled; wavefronts of unused slots
     // leave after the barrier
     Frame<K> f;
-    const bool live = cluster_frame<K, true>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f);
+    const bool live = cluster_frame<K, true>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f, wave);
     if (TAB) {
         if (threadIdx.x < 16) {
             tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
@@ -1536,7 +1549,7 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     if (COUNT) {
         for (int off = 32; off > 0; off >>= 1) npair += __shfl_xor(npair, off);
         if ((threadIdx.x & 63) == 0 && npair) {
-            const int slot = (int)((blockIdx.x * 4u + (threadIdx.x >> 6)) & (kPairSlots - 1));
+            const int slot = (int)((blockIdx.x * (unsigned)WPB + (threadIdx.x >> 6)) & (kPairSlots - 1));
             atomicAdd(pair_count + slot * kPairStride, (unsigned long long)npair);
         }
     }
