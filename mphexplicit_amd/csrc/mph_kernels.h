@@ -1470,8 +1470,11 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     const int2 m = cmap[entry];
     const int flag0 = any_flag[0], flag1 = any_flag[1];
     asm volatile("" ::"s"(m.x), "s"(m.y), "s"(flag0), "s"(flag1));
-    if (flag1 != 0 || (speculative && flag0 != 0) || m.x < 0) return;
-    const CaseConst& C = cases[m.x];
+    if (flag1 != 0 || (speculative && flag0 != 0)) return;      // the same for the whole workgroup
+    // a padding entry (m.x < 0) can share a workgroup with a real one: its wavefronts wait at
+    // the table's barrier below with the others and leave after it
+    const bool pad = m.x < 0;
+    const CaseConst& C = cases[pad ? 0 : m.x];
     // the InteractionRatio table: only the n^A sums weigh pairs by it
     constexpr bool TAB = ((MODE & 4) == 0 || SURF) && !((MODE & 2) != 0 && !SURF);
     __shared__ double tab[32];                // [0,16): mu_ij, [16,32): InteractionRatio
@@ -1480,7 +1483,7 @@ k_sweep_a(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
     // the frame's loads are in flight while the table is filled; wavefronts of unused slots
     // leave after the barrier
     Frame<K> f;
-    const bool live = cluster_frame<K, true>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f, wave);
+    const bool live = !pad && cluster_frame<K, true>(C, m.y, cl_info, cl_count, m.x, cl_nbr, cl_len, width, f, wave);
     if (TAB) {
         if (threadIdx.x < 16) {
             tab[threadIdx.x] = C.muij[threadIdx.x >> 2][threadIdx.x & 3];
