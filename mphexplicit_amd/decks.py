@@ -548,6 +548,16 @@ def small_two_fluid_3d(seed=13) -> Deck:
     return d
 
 
+def small_two_fluid_3d_no_surface_tension(seed=13) -> Deck:
+    """The 3D two-fluid block with non-unit interaction ratios and no surface tension: n^A
+    (DensityA) then weighs pairs by InteractionRatio but enters no force, so it is summed
+    only when the fields are read."""
+    d = small_two_fluid_3d(seed=seed)
+    d.data.surface_tension = (0.0,) * 4
+    d.name = "small_twofluid3d_nosurf"
+    return d
+
+
 def small_moving_wall_2d(seed=14) -> Deck:
     """2D fluid patch on a translating+rotating wall plate (types 2 and 3)."""
     rng = np.random.default_rng(seed)

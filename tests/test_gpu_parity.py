@@ -24,6 +24,9 @@ DECKS = {
     "rotatingwall3d": decks.small_rotating_wall_3d,
     "dambreak3d": decks.small_dam_break_3d,
     "twofluid3d": decks.small_two_fluid_3d,
+    # non-unit interaction ratios without surface tension: DensityA is summed when read, by
+    # sweep A with its InteractionRatio table in eight-wavefront workgroups
+    "twofluid3d_nosurf": decks.small_two_fluid_3d_no_surface_tension,
     # every width above eight cut-offs: the list build's bounding-box test is on in 3D
     "wide3d": decks.wide_dam_break_3d,
     # fully periodic drifting fluid: cluster members cross the faces between rebuilds while
@@ -119,7 +122,7 @@ def test_first_step_parity(solver_cls, name):
 
 
 @pytest.mark.parametrize("name", ["droplet2d", "movingwall2d", "bridge2d", "squaredrop2d", "twofluid3d", "wide3d",
-                                  "periodicflow3d", "rotatingwall3d", "narrowperiodic3d"])
+                                  "periodicflow3d", "rotatingwall3d", "narrowperiodic3d", "twofluid3d_nosurf"])
 def test_multi_step_parity(solver_cls, name):
     print("case", name, flush=True)
     d = DECKS[name]()
