@@ -165,6 +165,18 @@ __device__ __forceinline__ double by_type(const double (&a)[kTypes], int t)
     return t == 0 ? a0 : (t == 1 ? a1 : (t == 2 ? a2 : a3));
 }
 
+// The block-map entry of a sweep B workgroup, which walks its XCD's run of the map backwards:
+// the grid is the whole map (8 runs of `per` entries, XCD-interleaved, then 8 padding entries),
+// so sweep B starts on the clusters whose lists and records sweep A touched last (still in
+// that XCD's L2) and ends where the next step's sweep A begins.  On the 1.27M deck: FP64
+// 582-587 -> 589-591 steps/s (both sweeps about 1% faster), mixed precision 689-690 -> 712.
+__device__ __forceinline__ int reversed_entry()
+{
+    const int per = (int)gridDim.x / kXcds - 1;
+    const int j = (int)(blockIdx.x / kXcds);
+    return j < per ? (int)(blockIdx.x % kXcds) + kXcds * (per - 1 - j) : (int)blockIdx.x;
+}
+
 // The step's device flags, any_flag[0..1]:
 //   [0] some case rebuilds its lists this step (the Verlet trigger, raised by the move)
 //   [1] a list build of this step outgrew the reserved list width.  Every later kernel then
@@ -1677,7 +1689,7 @@ k_sweep_b(const CaseConst* __restrict__ cases, const int2* __restrict__ cmap,
         rebuilds[ncases] += 1;          // completed steps
         *any_flag = 0;
     }
-    const int2 m = cmap[blockIdx.x];
+    const int2 m = cmap[reversed_entry()];
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
     __shared__ double tab[48];                 // mu_ij, InteractionRatio, kv mu_ij

@@ -373,7 +373,7 @@ k_sweep_b_mixed(const CaseConst* __restrict__ cases, const int2* __restrict__ cm
         rebuilds[ncases] += 1;          // completed steps
         *any_flag = 0;
     }
-    const int2 m = cmap[blockIdx.x];
+    const int2 m = cmap[reversed_entry()];     // the map walked backwards, as k_sweep_b does
     if (m.x < 0) return;
     const CaseConst& C = cases[m.x];
     __shared__ float tabf[48];                 // mu_ij, InteractionRatio, kv mu_ij
